@@ -1,0 +1,125 @@
+"""C ABI surface and configuration parsing (host only, no GPU needed)."""
+import ctypes as C
+import json
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import vtp
+from vtp import _abi
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "vtp.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(vtp_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(lib):
+    names = declared_symbols()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(lib, name), f"libvtp.so does not export {name}"
+    assert set(names) == set(_abi.SIGNATURES), "ctypes signature table out of sync with include/vtp.h"
+
+
+def test_version_string(lib):
+    assert b"gfx950" in lib.vtp_version()
+
+
+def test_create_without_gpu_fails_cleanly(lib):
+    from conftest import gpu_available
+    if gpu_available():
+        pytest.skip("a GPU is present")
+    p = vtp.params_default("PBF")
+    h = C.c_void_p()
+    st = lib.vtp_create_with_params(C.byref(h), C.byref(p))
+    assert st == _abi.VTP_ENODEV
+    assert h.value is None
+    assert b"device" in lib.vtp_last_error(None)
+
+
+def test_null_context_calls_are_rejected(lib):
+    assert lib.vtp_step(None, 1) == _abi.VTP_EINVAL
+    assert lib.vtp_initialize(None) == _abi.VTP_EINVAL
+    assert lib.vtp_particle_count(None) == 0
+    lib.vtp_destroy(None)
+
+
+@pytest.mark.parametrize("solver", ["PBF", "DFSPH", "IMM", "IMMCT"])
+def test_template_defaults(solver):
+    p = vtp.params_default(solver)
+    assert p.solver == _abi.SOLVER_IDS[solver]
+    assert p.max_neighbor_num == 60
+    # gravity is (0,-9.8,0) unless Optional.enable (PBFSolver.cpp:53-58); templates have enable=false
+    assert list(p.gravity) == [0.0, np.float32(-9.8), 0.0]
+    assert p.max_solver_iterations == 100
+
+
+def test_pbf_template_values():
+    p = vtp.params_default("PBF")
+    assert np.isclose(p.time_step, 0.01) and np.isclose(p.particle_radius, 0.075)
+    assert p.iteration_num == 10 and np.isclose(p.xsph_k, 0.02)
+    assert list(p.sim_space_lb) == [-1, -1, -1] and list(p.sim_space_size) == [2, 2, 2]
+
+
+def test_immct_solvent_viscosity_fills_every_phase():
+    p = vtp.params_default("IMMCT")
+    assert p.phase_num == 2
+    assert np.allclose(list(p.phase_rest_viscosity)[:2], [0.01, 0.01])
+    assert np.isclose(p.solution_vis_base, 12) and np.isclose(p.relaxation_time, 0.001)
+    assert p.polymer_phase == 1
+
+
+def test_gravity_override_when_enabled():
+    cfg = vtp.scenes.pbf_config()
+    cfg["PBF"]["Optional"] = {"enable": True, "gravity": [1.0, 2.0, 3.0]}
+    p = vtp.params_from_json("PBF", json.dumps(cfg))
+    assert list(p.gravity) == [1.0, 2.0, 3.0]
+
+
+@pytest.mark.parametrize("key", ["timeStep", "iterationNum", "XSPH_k", "bPartRestDensity"])
+def test_missing_required_key_is_named(key):
+    cfg = vtp.scenes.pbf_config()
+    del cfg["PBF"]["Required"][key]
+    with pytest.raises(vtp.VtpError, match=f"missing Required key: {key}"):
+        vtp.params_from_json("PBF", json.dumps(cfg))
+
+
+def test_missing_optional_key_is_an_error():
+    cfg = vtp.scenes.pbf_config()
+    del cfg["PBF"]["Optional"]["gravity"]
+    with pytest.raises(vtp.VtpError, match="Optional key: gravity"):
+        vtp.params_from_json("PBF", json.dumps(cfg))
+
+
+def test_missing_domain_and_bad_json():
+    with pytest.raises(vtp.VtpError, match="main domain: DFSPH"):
+        vtp.params_from_json("DFSPH", json.dumps(vtp.scenes.pbf_config()))
+    with pytest.raises(vtp.VtpError, match="parse error"):
+        vtp.params_from_json("PBF", "{\"PBF\": [1, 2,}")
+
+
+def test_phase_size_mismatch_rejected():
+    cfg = vtp.scenes.immct_config(three_phase=False)
+    cfg["IMMCT"]["Required"]["phaseRestColor"] = [[1, 2, 3]]
+    with pytest.raises(vtp.VtpError, match="size not match"):
+        vtp.params_from_json("IMMCT", json.dumps(cfg))
+
+
+def test_three_phase_extension_keys():
+    cfg = vtp.scenes.immct_config(three_phase=True)
+    p = vtp.params_from_json("IMMCT", json.dumps(cfg))
+    assert p.phase_num == 3 and p.polymer_phase == 1
+    assert np.allclose(list(p.phase_rest_viscosity)[:3], [1e-3, 1e-3, 1e-2])
+    assert np.allclose(list(p.phase_rest_density)[:3], [1000, 800, 1200])
+
+
+def test_cfl_extension():
+    cfg = vtp.scenes.dfsph_config(cfl=True)
+    p = vtp.params_from_json("DFSPH", json.dumps(cfg))
+    assert p.cfl_enable == 1 and np.isclose(p.cfl_number, 0.4) and np.isclose(p.dt_max, 0.002)

@@ -1,0 +1,105 @@
+"""DFSPH step on the GPU vs the FP64 oracle (DFSPHCudaImpl.cu, DFSPHSolver.cpp:339-406).
+
+Checks: iteration counts of both pressure solves equal the oracle's, exit
+residuals agree (FP64 device reduction vs FP64 host sum, rel 1e-4 since the
+per-particle deltas are FP32), and the fields after one step agree with the
+oracle within 1e-5 (L2 norm-wise relative; max norm 1e-5 for x, v, m, V).
+Derived quantities with cancellation (delta = max(0, cr - 1 + ...),
+kappa, acceleration = (v* - v)/dt) are compared at 1e-4.
+"""
+import numpy as np
+import pytest
+
+import pyoracle
+import vtp
+from conftest import rel_err
+from vtp import scenes
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL = {"pos": 1e-5, "vel": 1e-5, "mass": 1e-5, "volume": 1e-5, "rest_density": 1e-5,
+             "compression_ratio": 1e-5, "df_alpha": 1e-5, "vel_adv": 1e-5, "surface_normal": 1e-4,
+             "delta_cr": 1e-4, "kappa_incomp": 1e-4, "acc": 1e-4}
+
+
+def l2_rel(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / nb) if nb else float(np.linalg.norm(a))
+
+
+def _pair(wl):
+    params = vtp.params_from_json("DFSPH", wl.config_json())
+    ctx = vtp.Context(params)
+    for o in wl.objects:
+        ctx.add_particles(o.pos, o.mat, o.vel, None, o.scene)
+    ctx.initialize()
+    orc = pyoracle.Oracle.from_workload(params, wl)
+    orc.initialize()
+    return ctx, orc
+
+
+def _compare(ctx, orc, fields=FIELD_TOL):
+    ids = vtp._abi.FIELDS
+    res = {}
+    bad = {}
+    for f, tol in fields.items():
+        g = ctx.read(f)
+        r = orc.read(ids[f][0])
+        e = l2_rel(g, r)
+        res[f] = e
+        if not e <= tol:
+            bad[f] = e
+    assert not bad, f"over tolerance: {bad} (all {res})"
+    return res
+
+
+def test_dfsph_init_matches():
+    wl = scenes.cfg1_dfsph(n_side=(16, 16, 16))
+    ctx, orc = _pair(wl)
+    _compare(ctx, orc, {"pos": 1e-7, "vel": 1e-7, "mass": 1e-6, "volume": 1e-6, "rest_density": 1e-6})
+
+
+@pytest.mark.parametrize("tension", [0.0, 0.05])
+def test_dfsph_one_step(tension):
+    wl = scenes.cfg1_dfsph(n_side=(24, 24, 24))
+    wl.config["DFSPH"]["Required"]["surfaceTensionCoefficient"] = tension
+    ctx, orc = _pair(wl)
+    ctx.step(1)
+    orc.step(1)
+    gs, rs = ctx.stats(), orc.stats()
+    assert gs["div_iterations"] == rs["div_iterations"]
+    assert gs["incomp_iterations"] == rs["incomp_iterations"]
+    assert abs(gs["div_residual"] - rs["div_residual"]) <= 1e-4 * abs(rs["div_residual"]) + 1e-12
+    assert abs(gs["incomp_residual"] - rs["incomp_residual"]) <= 1e-4 * abs(rs["incomp_residual"]) + 1e-12
+    _compare(ctx, orc)
+    for f in ("pos", "vel", "mass", "volume"):
+        assert rel_err(ctx.read(f), orc.read(vtp._abi.FIELDS[f][0])) <= 1e-5
+
+
+def test_dfsph_five_steps_iteration_counts():
+    wl = scenes.cfg1_dfsph(n_side=(20, 20, 20))
+    ctx, orc = _pair(wl)
+    for _ in range(5):
+        ctx.step(1)
+        orc.step(1)
+        gs, rs = ctx.stats(), orc.stats()
+        assert (gs["div_iterations"], gs["incomp_iterations"]) == (rs["div_iterations"], rs["incomp_iterations"])
+    _compare(ctx, orc, {"pos": 1e-5, "vel": 1e-4})
+
+
+def test_dfsph_iteration_cap_sets_crash():
+    wl = scenes.cfg1_dfsph(n_side=(12, 12, 12))
+    wl.config["DFSPH"]["Required"]["incompThreshold"] = 0.0   # never converges
+    wl.config["DFSPH"]["Extensions"] = {"maxSolverIterations": 7}
+    ctx, orc = _pair(wl)
+    with pytest.raises(vtp.VtpError, match="VTP_ECRASH"):
+        ctx.step(1)
+    st = ctx.stats()
+    assert st["incomp_iterations"] == 8          # cap + 1, as iter > cap breaks
+    assert st["crash_flags"] & vtp._abi.CRASH_INCOMP_ITER_CAP
+    orc.step(1)
+    assert orc.stats()["incomp_iterations"] == 8
+    with pytest.raises(vtp.VtpError, match="crashed"):
+        ctx.step(1)

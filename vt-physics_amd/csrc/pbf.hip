@@ -1,0 +1,303 @@
+// pbf.hip - position-based fluids step (PBFCudaImpl.cu:34-281, PBFSolver.cpp:326-376).
+//
+// One tick = predict+hash -> sort -> reorder -> neighbour lists ->
+// K x (density+lambda pass, delta-x pass) -> XSPH pass.  Compared with the
+// reference's 8 kernels:
+//   * density/error and lambda share one neighbour pass (both only read
+//     positions and masses); results accumulate in registers, not in global
+//     memory (the reference does `d_data->x[p_i] += ...` per neighbour);
+//   * delta-x and apply_dx are one pass writing into a second position
+//     buffer (Jacobi semantics preserved: every delta-x reads the positions
+//     of the previous iteration), XSPH and its apply likewise;
+//   * the predict (apply_ext_force) is fused with the cell hash.
+// Boundary particles never move: their positions sit in both ping-pong
+// buffers, so the passes skip them entirely.
+#include "vtp_internal.h"
+
+namespace vtp {
+
+struct PbfK {
+    Cubic k;
+    float dt, inv_dt;
+    float rho0, inv_rho0;
+    float xsph_k;
+    float gx, gy, gz;
+};
+
+static PbfK pbf_constants(const Ctx &c) {
+    PbfK q;
+    q.k = c.kern;
+    q.dt = c.p.time_step;
+    q.inv_dt = c.inv_dt;
+    q.rho0 = c.p.f_rest_density;
+    q.inv_rho0 = 1.0f / c.p.f_rest_density;
+    q.xsph_k = c.p.xsph_k;
+    q.gx = c.p.gravity[0];
+    q.gy = c.p.gravity[1];
+    q.gz = c.p.gravity[2];
+    return q;
+}
+
+// apply_ext_force_cuda (PBFCudaImpl.cu:146-157) + calcParticleHashValue.
+__global__ void k_pbf_predict_hash(float4 *__restrict__ x, float4 *__restrict__ v,
+                                   const int32_t *__restrict__ mat, const uint32_t *__restrict__ scene,
+                                   uint32_t n, PbfK q, Grid g, uint32_t *__restrict__ key,
+                                   uint32_t *__restrict__ val, uint32_t *__restrict__ status) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 xi = x[i];
+    if (mat[i] == VTP_MAT_FLUID) {
+        float4 vi = v[i];
+        vi.x += q.gx * q.dt;
+        vi.y += q.gy * q.dt;
+        vi.z += q.gz * q.dt;
+        xi.x += vi.x * q.dt;
+        xi.y += vi.y * q.dt;
+        xi.z += vi.z * q.dt;
+        v[i] = vi;
+        x[i] = xi;
+    }
+    uint32_t k = cell_key(xyz(xi), scene[i], g);
+    bool finite = isfinite(xi.x) && isfinite(xi.y) && isfinite(xi.z);
+    if (!finite) k = g.dead_key;
+    key[i] = k;
+    val[i] = i;
+    if (k == g.dead_key) {
+        atomicAdd(&status[kStDead], 1u);
+        atomicOr(&status[kStCrash], finite ? (uint32_t)VTP_CRASH_OUT_OF_GRID : (uint32_t)VTP_CRASH_NAN);
+    }
+}
+
+// compute_sph_density_and_error_cuda + update_lamb_cuda (PBFCudaImpl.cu:70-122):
+//   rho_i = max(sum_j m_j W_ij, rho0),  C_i = rho_i/rho0 - 1,
+//   S_i = sum_j |gradW_ij/rho0|^2 + |sum_j gradW_ij / rho0|^2,
+//   lambda_i = -C_i / (S_i + 1e-6)          (mass-free gradient, as the reference).
+// Boundary lambda is defined as 0 (the reference never writes it).
+__global__ __launch_bounds__(256) void k_pbf_lambda(const float4 *__restrict__ x,
+                                                    const int32_t *__restrict__ mat,
+                                                    const uint32_t *__restrict__ nbr,
+                                                    const uint32_t *__restrict__ cnt, uint32_t n,
+                                                    uint32_t stride, PbfK q, float *__restrict__ lam,
+                                                    float *__restrict__ dens, float *__restrict__ errc) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (mat[i] != VTP_MAT_FLUID) {
+        // slots are permuted every tick, so non-fluid slots are written too
+        lam[i] = 0.0f;
+        dens[i] = 0.0f;
+        errc[i] = 0.0f;
+        return;
+    }
+    const float4 xi = x[i];
+    const uint32_t w = cnt[i];
+    const uint32_t nn = w & ~kDeadBit;
+    float rho = (w & kDeadBit) ? 0.0f : xi.w * q.k.w0;
+    float gsx = 0, gsy = 0, gsz = 0, s2 = 0;
+    for (uint32_t t = 0; t < nn; ++t) {
+        uint32_t j = nbr[(size_t)t * stride + i];
+        float4 xj = x[j];
+        float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        float W, f;
+        cubic_w_grad(r2, q.k, W, f);
+        rho = fmaf(xj.w, W, rho);
+        float ax = dx * f, ay = dy * f, az = dz * f;
+        gsx += ax;
+        gsy += ay;
+        gsz += az;
+        s2 += ax * ax + ay * ay + az * az;
+    }
+    rho = fmaxf(rho, q.rho0);
+    float C = rho * q.inv_rho0 - 1.0f;
+    float ir2 = q.inv_rho0 * q.inv_rho0;
+    float S = (s2 + gsx * gsx + gsy * gsy + gsz * gsz) * ir2;
+    lam[i] = -C / (S + 1e-6f);
+    dens[i] = rho;
+    errc[i] = C;
+}
+
+// compute_dx_cuda + apply_dx_cuda (PBFCudaImpl.cu:124-170):
+//   dx_i = (1/rho0) sum_j (lambda_i + lambda_j) gradW_ij,
+//   x_i <- x_i + dx_i,  v_i <- v_i + dx_i / dt.
+__global__ __launch_bounds__(256) void k_pbf_dx(const float4 *__restrict__ x, float4 *__restrict__ xo,
+                                                float4 *__restrict__ v, const int32_t *__restrict__ mat,
+                                                const uint32_t *__restrict__ nbr,
+                                                const uint32_t *__restrict__ cnt, uint32_t n, uint32_t stride,
+                                                PbfK q, const float *__restrict__ lam) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (mat[i] != VTP_MAT_FLUID) return;
+    const float4 xi = x[i];
+    const float li = lam[i];
+    const uint32_t nn = cnt[i] & ~kDeadBit;
+    float ax = 0, ay = 0, az = 0;
+    for (uint32_t t = 0; t < nn; ++t) {
+        uint32_t j = nbr[(size_t)t * stride + i];
+        float4 xj = x[j];
+        float lj = lam[j];
+        float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        float s = cubic_grad_factor(r2, q.k) * (li + lj);
+        ax = fmaf(dx, s, ax);
+        ay = fmaf(dy, s, ay);
+        az = fmaf(dz, s, az);
+    }
+    ax *= q.inv_rho0;
+    ay *= q.inv_rho0;
+    az *= q.inv_rho0;
+    xo[i] = make_float4(xi.x + ax, xi.y + ay, xi.z + az, xi.w);
+    float4 vi = v[i];
+    vi.x += ax * q.inv_dt;
+    vi.y += ay * q.inv_dt;
+    vi.z += az * q.inv_dt;
+    v[i] = vi;
+}
+
+// XSPH_cuda + apply_dx_cuda (PBFCudaImpl.cu:172-194, 267-281):
+//   dv_i = k sum_j V_j (v_j - v_i) W_ij,  dx_i = dv_i dt,
+//   x_i <- x_i + dx_i,  v_i <- v_i + dx_i / dt.
+// Writes every slot of the output buffers (boundary slots are copied), so the
+// state buffers X[0]/V[0] are complete after the pass.
+__global__ __launch_bounds__(256) void k_pbf_xsph(const float4 *__restrict__ x, float4 *__restrict__ xo,
+                                                  const float4 *__restrict__ v, float4 *__restrict__ vo,
+                                                  const int32_t *__restrict__ mat,
+                                                  const uint32_t *__restrict__ nbr,
+                                                  const uint32_t *__restrict__ cnt, uint32_t n,
+                                                  uint32_t stride, PbfK q) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 xi = x[i];
+    const float4 vi = v[i];
+    if (mat[i] != VTP_MAT_FLUID) {
+        xo[i] = xi;
+        vo[i] = vi;
+        return;
+    }
+    const uint32_t nn = cnt[i] & ~kDeadBit;
+    float ax = 0, ay = 0, az = 0;
+    for (uint32_t t = 0; t < nn; ++t) {
+        uint32_t j = nbr[(size_t)t * stride + i];
+        float4 xj = x[j];
+        float4 vj = v[j];
+        float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        float s = vj.w * cubic_w(r2, q.k);
+        ax = fmaf(vj.x - vi.x, s, ax);
+        ay = fmaf(vj.y - vi.y, s, ay);
+        az = fmaf(vj.z - vi.z, s, az);
+    }
+    float c = q.xsph_k * q.dt;
+    float dx = ax * c, dy = ay * c, dz = az * c;
+    xo[i] = make_float4(xi.x + dx, xi.y + dy, xi.z + dz, xi.w);
+    vo[i] = make_float4(vi.x + dx * q.inv_dt, vi.y + dy * q.inv_dt, vi.z + dz * q.inv_dt, vi.w);
+}
+
+// init_cuda + compute_rigid_particle_volume_cuda (PBFCudaImpl.cu:34-68):
+// fluid V = V0, m = rho0_f V0; boundary V = 1 / sum_{j same material} W_ij
+// (self included), m = rho0_b V.
+__global__ void k_pbf_init(float4 *__restrict__ x, float4 *__restrict__ v, const int32_t *__restrict__ mat,
+                           const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ cnt, uint32_t n,
+                           uint32_t stride, Cubic k, float V0, float rho0_f, float rho0_b) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t mi = mat[i];
+    float vol = V0, mass = rho0_f * V0;
+    if (mi == VTP_MAT_BOUNDARY) {
+        const float4 xi = x[i];
+        const uint32_t w = cnt[i];
+        const uint32_t nn = w & ~kDeadBit;
+        float delta = (w & kDeadBit) ? 0.0f : k.w0;
+        for (uint32_t t = 0; t < nn; ++t) {
+            uint32_t j = nbr[(size_t)t * stride + i];
+            if (mat[j] != mi) continue;
+            float4 xj = x[j];
+            float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            delta += cubic_w(dx * dx + dy * dy + dz * dz, k);
+        }
+        vol = 1.0f / delta;
+        mass = rho0_b * vol;
+    }
+    x[i].w = mass;
+    v[i].w = vol;
+}
+
+// ------------------------------------------------------------------ host side
+void common_upload_and_search(Ctx &c);
+void launch_step_begin(Ctx &c);
+void common_reorder(Ctx &c, const float4 *xs, float4 *xd, float4 *xd2, const float4 *vs, float4 *vd,
+                    float4 *vd2);
+
+void pbf_initialize(Ctx &c) {
+    const uint32_t n = c.n;
+    c.lam.alloc(n);
+    c.dens.alloc(n);
+    c.err_c.alloc(n);
+    VTP_HIP(hipMemsetAsync(c.lam.p, 0, c.lam.bytes(), c.stream));
+    VTP_HIP(hipMemsetAsync(c.dens.p, 0, c.dens.bytes(), c.stream));
+    VTP_HIP(hipMemsetAsync(c.err_c.p, 0, c.err_c.bytes(), c.stream));
+    common_upload_and_search(c);   // state in X[0]/V[0], lists built
+    hipLaunchKernelGGL(k_pbf_init, dim3(div_up(n, 256)), dim3(256), 0, c.stream, c.X[0].p, c.V[0].p, c.mat.p,
+                       c.ugns.nbr.p, c.ugns.cnt.p, n, c.ugns.stride, c.kern, c.V0, c.p.f_rest_density,
+                       c.p.b_rest_density);
+    VTP_HIP(hipGetLastError());
+}
+
+static void pbf_enqueue_step(Ctx &c) {
+    const uint32_t n = c.n;
+    const PbfK q = pbf_constants(c);
+    Ugns &u = c.ugns;
+    const uint32_t nb = div_up(n, 256);
+    launch_step_begin(c);
+    c.timer.begin(c.stream, "pbf_predict_hash");
+    hipLaunchKernelGGL(k_pbf_predict_hash, dim3(nb), dim3(256), 0, c.stream, c.X[0].p, c.V[0].p, c.mat.p,
+                       c.scene.p, n, q, c.grid, u.key[0].p, u.val[0].p, c.status.p);
+    c.timer.end(c.stream);
+    u.sort_and_rank(c.orig.p, c.stream, c.timer);
+    c.timer.begin(c.stream, "pbf_reorder");
+    common_reorder(c, c.X[0].p, c.X[1].p, c.X[2].p, c.V[0].p, c.V[1].p, nullptr);
+    c.timer.end(c.stream);
+    u.build_neighbors(c.X[1].p, c.stream, c.timer);
+    float4 *cur = c.X[1].p, *nxt = c.X[2].p;
+    const uint32_t bl = c.block("pbf_lambda"), bd = c.block("pbf_dx"), bx = c.block("pbf_xsph");
+    for (uint32_t it = 0; it < c.p.iteration_num; ++it) {
+        c.timer.begin(c.stream, "pbf_lambda");
+        hipLaunchKernelGGL(k_pbf_lambda, dim3(div_up(n, bl)), dim3(bl), 0, c.stream, cur, c.mat.p, u.nbr.p,
+                           u.cnt.p, n, u.stride, q, c.lam.p, c.dens.p, c.err_c.p);
+        c.timer.end(c.stream);
+        c.timer.begin(c.stream, "pbf_dx");
+        hipLaunchKernelGGL(k_pbf_dx, dim3(div_up(n, bd)), dim3(bd), 0, c.stream, cur, nxt, c.V[1].p, c.mat.p,
+                           u.nbr.p, u.cnt.p, n, u.stride, q, c.lam.p);
+        c.timer.end(c.stream);
+        std::swap(cur, nxt);
+    }
+    c.timer.begin(c.stream, "pbf_xsph");
+    hipLaunchKernelGGL(k_pbf_xsph, dim3(div_up(n, bx)), dim3(bx), 0, c.stream, cur, c.X[0].p, c.V[1].p,
+                       c.V[0].p, c.mat.p, u.nbr.p, u.cnt.p, n, u.stride, q);
+    c.timer.end(c.stream);
+}
+
+void pbf_capture_reset(Ctx &c) {
+    if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
+    if (c.graph) (void)hipGraphDestroy(c.graph);
+    c.graph_exec = nullptr;
+    c.graph = nullptr;
+}
+
+void pbf_step(Ctx &c) {
+    if (c.opt_graph && !c.timer.enabled) {
+        if (!c.graph_exec) {
+            VTP_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+            pbf_enqueue_step(c);
+            hipGraph_t g = nullptr;
+            VTP_HIP(hipStreamEndCapture(c.stream, &g));
+            c.graph = g;
+            VTP_HIP(hipGraphInstantiate(&c.graph_exec, c.graph, nullptr, nullptr, 0));
+        }
+        VTP_HIP(hipGraphLaunch(c.graph_exec, c.stream));
+    } else {
+        pbf_enqueue_step(c);
+    }
+    VTP_HIP(hipGetLastError());
+}
+
+}  // namespace vtp

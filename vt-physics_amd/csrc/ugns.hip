@@ -1,0 +1,353 @@
+// ugns.hip - uniform-grid neighbour search for MI355X.
+//
+// Replaces UGNSCudaImpl.cu:96-227 (resetDevPtr, calcParticleHashValue,
+// sortByHashValue = 2x thrust::sort_by_key, findCellRange, findNeighbors).
+//
+// Differences in structure (results are identical):
+//  * particle state is kept in cell-sorted slot order and physically
+//    reordered after every sort, so neighbour gathers hit contiguous memory;
+//  * the sort is a stable LSD radix sort written here: 8-bit digits, a
+//    per-tile digit histogram, a per-digit spine scan and a scatter whose
+//    stable in-tile ranks come from wavefront ballots (one 64-bit ballot per
+//    key bit gives every lane the mask of lanes holding the same digit);
+//  * because the input is the previous step's slot order rather than attach
+//    order, equal cell ids are put in attach-id order by a per-cell rank pass,
+//    which reproduces thrust's stable (cellId, attachId) order exactly;
+//  * neighbour lists hold slot indices in an ELL layout nbr[t*stride + slot]
+//    (t-major: lane i of a wavefront reads word i, fully coalesced) without
+//    the self entry; the reference layout is rebuilt on request.
+#include "vtp_internal.h"
+
+namespace vtp {
+
+constexpr int kRsThreads = 256;
+constexpr int kRsRounds = 16;                   // keys per lane per tile
+constexpr int kRsWaveChunk = 64 * kRsRounds;    // 1024 keys per wavefront
+constexpr int kRsTile = kRsThreads * kRsRounds; // 4096 keys per workgroup
+
+__global__ void k_fill_u32(uint32_t *a, uint32_t n, uint32_t v) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// Digit histogram of one tile.  Keys of consecutive slots usually share a
+// digit, so lanes first merge equal digits with ballots and only the leader
+// of each group touches LDS.
+__global__ __launch_bounds__(kRsThreads) void k_radix_hist(const uint32_t *__restrict__ keys, uint32_t n,
+                                                           int shift, uint32_t *__restrict__ spine,
+                                                           uint32_t tiles) {
+    __shared__ uint32_t h[256];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    h[tid] = 0;
+    __syncthreads();
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t base = blockIdx.x * kRsTile + w * kRsWaveChunk;
+    for (int r = 0; r < kRsRounds; ++r) {
+        uint32_t i = base + r * 64 + lane;
+        bool valid = i < n;
+        uint32_t d = valid ? (keys[i] >> shift) & 255u : 0u;
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            uint64_t bb = __ballot((d >> b) & 1u);
+            m &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        if (valid && (m & lt) == 0) atomicAdd(&h[d], (uint32_t)__popcll(m));
+    }
+    __syncthreads();
+    spine[(uint32_t)tid * tiles + blockIdx.x] = h[tid];
+}
+
+// One workgroup per digit: exclusive scan of that digit's per-tile counts,
+// total into dtot[d].
+__global__ __launch_bounds__(kRsThreads) void k_radix_spine(uint32_t *__restrict__ spine, uint32_t tiles,
+                                                            uint32_t *__restrict__ dtot) {
+    __shared__ uint32_t s[kRsThreads];
+    const int tid = threadIdx.x;
+    uint32_t *row = spine + (uint32_t)blockIdx.x * tiles;
+    const uint32_t per = (tiles + kRsThreads - 1) / kRsThreads;
+    const uint32_t b = tid * per, e = min(b + per, tiles);
+    uint32_t sum = 0;
+    for (uint32_t t = b; t < e; ++t) sum += row[t];
+    s[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < kRsThreads; off <<= 1) {
+        uint32_t v = tid >= off ? s[tid - off] : 0u;
+        __syncthreads();
+        s[tid] += v;
+        __syncthreads();
+    }
+    uint32_t run = s[tid] - sum;  // exclusive prefix of this thread's chunk
+    for (uint32_t t = b; t < e; ++t) {
+        uint32_t c = row[t];
+        row[t] = run;
+        run += c;
+    }
+    if (tid == kRsThreads - 1) dtot[blockIdx.x] = s[tid];
+}
+
+// Stable scatter of one tile.  Wavefront w owns keys [w*1024, (w+1)*1024) of
+// the tile, processed in 16 rounds of 64 in index order, so the order
+// (wave, round, lane) equals the input order and ranks are stable.
+__global__ __launch_bounds__(kRsThreads) void k_radix_scatter(const uint32_t *__restrict__ kin,
+                                                              const uint32_t *__restrict__ vin,
+                                                              uint32_t *__restrict__ kout,
+                                                              uint32_t *__restrict__ vout, uint32_t n,
+                                                              int shift, const uint32_t *__restrict__ spine,
+                                                              const uint32_t *__restrict__ dtot,
+                                                              uint32_t tiles) {
+    __shared__ uint32_t wc[4][256];   // per-wave running digit counts
+    __shared__ uint32_t toff[256];    // global offset of each digit for this tile
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+
+    // exclusive scan of the 256 digit totals
+    toff[tid] = dtot[tid];
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        uint32_t v = tid >= off ? toff[tid - off] : 0u;
+        __syncthreads();
+        toff[tid] += v;
+        __syncthreads();
+    }
+    uint32_t incl = toff[tid];
+    __syncthreads();
+    toff[tid] = incl - dtot[tid] + spine[(uint32_t)tid * tiles + blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) wc[q][tid] = 0;
+    __syncthreads();
+
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t base = blockIdx.x * kRsTile + w * kRsWaveChunk;
+    uint32_t kr[kRsRounds], vr[kRsRounds], rr[kRsRounds];
+#pragma unroll
+    for (int r = 0; r < kRsRounds; ++r) {
+        uint32_t i = base + r * 64 + lane;
+        bool valid = i < n;
+        uint32_t k = valid ? kin[i] : 0u;
+        uint32_t d = (k >> shift) & 255u;
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            uint64_t bb = __ballot((d >> b) & 1u);
+            m &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        uint32_t rank = (uint32_t)__popcll(m & lt);
+        uint32_t prev = wc[w][d];
+        // the leader's store depends on `prev`, so every lane's read of this
+        // round completes before the running count moves on
+        if (valid && rank == 0) wc[w][d] = prev + (uint32_t)__popcll(m);
+        kr[r] = k;
+        vr[r] = valid ? vin[i] : 0u;
+        rr[r] = prev + rank;
+    }
+    __syncthreads();
+    {   // per digit: exclusive prefix over the four wavefronts
+        uint32_t s = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint32_t t = wc[q][tid];
+            wc[q][tid] = s;
+            s += t;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRsRounds; ++r) {
+        uint32_t i = base + r * 64 + lane;
+        if (i < n) {
+            uint32_t d = (kr[r] >> shift) & 255u;
+            uint32_t pos = toff[d] + wc[w][d] + rr[r];
+            kout[pos] = kr[r];
+            vout[pos] = vr[r];
+        }
+    }
+}
+
+// cellStart / cellEnd (inclusive) of every non-empty cell, UGNSCudaImpl.cu:144-166.
+__global__ void k_cell_ranges(const uint32_t *__restrict__ key, uint32_t n, uint32_t *__restrict__ start,
+                              uint32_t *__restrict__ end) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t k = key[i];
+    if (i == 0 || key[i - 1] != k) start[k] = i;
+    if (i == n - 1 || key[i + 1] != k) end[k] = i;
+}
+
+__global__ void k_gather_u32(const uint32_t *__restrict__ src, const uint32_t *__restrict__ idx, uint32_t n,
+                             uint32_t *__restrict__ dst) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
+// Within one cell the radix sort leaves particles in previous-slot order;
+// rank them by attach id so the final order is (cellId, attachId), the order
+// thrust's stable sort of attach-ordered input produces.  Off-grid particles
+// share the dead key and keep the stable order.
+__global__ void k_cell_fixup(const uint32_t *__restrict__ key, const uint32_t *__restrict__ val,
+                             const uint32_t *__restrict__ so, const uint32_t *__restrict__ start,
+                             const uint32_t *__restrict__ end, uint32_t n, uint32_t dead_key,
+                             uint32_t *__restrict__ perm, uint32_t *__restrict__ orig_new) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t k = key[i];
+    uint32_t my = so[i];
+    uint32_t dst = i;
+    if (k != dead_key) {
+        uint32_t s = start[k], e = end[k];
+        uint32_t rank = 0;
+        for (uint32_t j = s; j <= e; ++j) rank += so[j] < my ? 1u : 0u;
+        dst = s + rank;
+    }
+    perm[dst] = val[i];
+    orig_new[dst] = my;
+}
+
+// Neighbour lists, UGNSCudaImpl.cu:168-203: for the 27 cells in the
+// reference's offset order (x outermost, z innermost,
+// UniformGridNeighborSearch.cpp:33-60), every other particle of the cell in
+// slot order that passes 0 < |x_i - x_j| <= h, capped at M-1 entries.
+__global__ __launch_bounds__(256) void k_build_neighbors(const float4 *__restrict__ x,
+                                                         const uint32_t *__restrict__ key,
+                                                         const uint32_t *__restrict__ start,
+                                                         const uint32_t *__restrict__ end, Grid g, uint32_t n,
+                                                         uint32_t stride, uint32_t cap,
+                                                         uint32_t *__restrict__ nbr, uint32_t *__restrict__ cnt) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t k = key[i];
+    if (k >= g.dead_key) {
+        cnt[i] = kDeadBit;
+        return;
+    }
+    const uint32_t gxy = (uint32_t)(g.gx * g.gy);
+    uint32_t scene = k / g.cells;
+    uint32_t local = k - scene * g.cells;
+    int cz = (int)(local / gxy);
+    uint32_t rem = local - (uint32_t)cz * gxy;
+    int cy = (int)(rem / (uint32_t)g.gx);
+    int cx = (int)(rem - (uint32_t)cy * (uint32_t)g.gx);
+    const uint32_t sbase = scene * g.cells;
+    float3 p = xyz(x[i]);
+    uint32_t c = 0;
+    for (int ox = -1; ox <= 1; ++ox) {
+        int X = cx + ox;
+        if (X < 0 || X >= g.gx) continue;
+        for (int oy = -1; oy <= 1; ++oy) {
+            int Y = cy + oy;
+            if (Y < 0 || Y >= g.gy) continue;
+            for (int oz = -1; oz <= 1; ++oz) {
+                int Z = cz + oz;
+                if (Z < 0 || Z >= g.gz) continue;
+                uint32_t cid = sbase + (uint32_t)Z * gxy + (uint32_t)Y * (uint32_t)g.gx + (uint32_t)X;
+                uint32_t s = start[cid];
+                if (s == kEmpty) continue;
+                uint32_t e = end[cid];
+                for (uint32_t j = s; j <= e; ++j) {
+                    if (j == i) continue;
+                    float d2 = dist2_exact(p, xyz(x[j]));
+                    if (is_neighbor(d2, g.nbr_d2_max) && c < cap) {
+                        nbr[(size_t)c * stride + i] = j;
+                        ++c;
+                    }
+                }
+            }
+        }
+    }
+    cnt[i] = c;
+}
+
+// Reference-layout export (UniformGridNeighborSearch.hpp:28-63): attach ids,
+// self first, UINT32_MAX padding.
+__global__ void k_export_neighbors(const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ cnt,
+                                   const uint32_t *__restrict__ orig, uint32_t n, uint32_t stride, uint32_t M,
+                                   uint32_t *__restrict__ out_lists, uint32_t *__restrict__ out_num) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t p = orig[i];
+    uint32_t w = cnt[i];
+    uint32_t *row = out_lists + (size_t)p * M;
+    if (w & kDeadBit) {
+        out_num[p] = 0;
+        for (uint32_t t = 0; t < M; ++t) row[t] = kEmpty;
+        return;
+    }
+    uint32_t c = w & ~kDeadBit;
+    row[0] = p;
+    for (uint32_t t = 0; t < c; ++t) row[1 + t] = orig[nbr[(size_t)t * stride + i]];
+    for (uint32_t t = c + 1; t < M; ++t) row[t] = kEmpty;
+    out_num[p] = c + 1;
+}
+
+// ---------------------------------------------------------------- host side
+static inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
+
+void Ugns::allocate(uint32_t n_, const Grid &g_, uint32_t max_neighbors) {
+    n = n_;
+    g = g_;
+    M = max_neighbors;
+    cap = M > 0 ? M - 1 : 0;
+    stride = (n + 63u) & ~63u;
+    uint32_t maxkey = g.dead_key;
+    key_bits = 1;
+    while (key_bits < 32 && (maxkey >> key_bits) != 0) ++key_bits;
+    passes = (key_bits + 7) / 8;
+    tiles = blocks_for(n, kRsTile);
+    for (int b = 0; b < 2; ++b) {
+        key[b].alloc(n);
+        val[b].alloc(n);
+    }
+    spine.alloc((size_t)256 * tiles);
+    dtot.alloc(256);
+    start.alloc((size_t)g.dead_key + 1);
+    end.alloc((size_t)g.dead_key + 1);
+    so.alloc(n);
+    perm.alloc(n);
+    orig_new.alloc(n);
+    nbr.alloc((size_t)cap * stride + 1);
+    cnt.alloc(n);
+}
+
+// Input: key[0], val[0] filled by the solver's hash kernel.
+// Output: sorted keys in sorted_key(), permutation perm (old slot of each new
+// slot) and orig_new (attach id of each new slot).
+void Ugns::sort_and_rank(const uint32_t *orig_old, hipStream_t s, StageTimer &tm) {
+    tm.begin(s, "ugns_sort");
+    int b = 0;
+    for (uint32_t p = 0; p < passes; ++p) {
+        int shift = (int)(8 * p);
+        hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kRsThreads), 0, s, key[b].p, n, shift, spine.p, tiles);
+        hipLaunchKernelGGL(k_radix_spine, dim3(256), dim3(kRsThreads), 0, s, spine.p, tiles, dtot.p);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kRsThreads), 0, s, key[b].p, val[b].p,
+                           key[b ^ 1].p, val[b ^ 1].p, n, shift, spine.p, dtot.p, tiles);
+        b ^= 1;
+    }
+    sorted = b;
+    tm.end(s);
+    tm.begin(s, "ugns_ranges");
+    const uint32_t ncell = g.dead_key + 1;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(ncell, 256)), dim3(256), 0, s, start.p, ncell, kEmpty);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(ncell, 256)), dim3(256), 0, s, end.p, ncell, kEmpty);
+    hipLaunchKernelGGL(k_cell_ranges, dim3(blocks_for(n, 256)), dim3(256), 0, s, key[b].p, n, start.p, end.p);
+    hipLaunchKernelGGL(k_gather_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, orig_old, val[b].p, n, so.p);
+    hipLaunchKernelGGL(k_cell_fixup, dim3(blocks_for(n, 256)), dim3(256), 0, s, key[b].p, val[b].p, so.p,
+                       start.p, end.p, n, g.dead_key, perm.p, orig_new.p);
+    tm.end(s);
+}
+
+void Ugns::build_neighbors(const float4 *x, hipStream_t s, StageTimer &tm) {
+    tm.begin(s, "ugns_neighbors");
+    hipLaunchKernelGGL(k_build_neighbors, dim3(blocks_for(n, 256)), dim3(256), 0, s, x, key[sorted].p, start.p,
+                       end.p, g, n, stride, cap, nbr.p, cnt.p);
+    tm.end(s);
+}
+
+void Ugns::export_reference(const uint32_t *orig, uint32_t *d_lists, uint32_t *d_num, hipStream_t s) {
+    hipLaunchKernelGGL(k_export_neighbors, dim3(blocks_for(n, 256)), dim3(256), 0, s, nbr.p, cnt.p, orig, n,
+                       stride, M, d_lists, d_num);
+}
+
+void launch_fill_u32(uint32_t *a, uint32_t n, uint32_t v, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, a, n, v);
+}
+
+}  // namespace vtp

@@ -1,0 +1,151 @@
+// vtp_device.h - device helpers shared by every solver: float3 arithmetic, the
+// FP32 decision functions of the uniform-grid search (cell coordinate and the
+// neighbour test, both bit-exact with the reference), the cubic-spline SPH
+// kernels and wavefront reductions.  gfx950 only (wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vtp {
+
+// Launch-side description of the uniform grid, passed by value.
+struct Grid {
+    float lb_x, lb_y, lb_z;   // simSpaceLB
+    float cell;               // gridCellSize == h (PBFSolver.cpp:89)
+    int gx, gy, gz;           // ceil(simSpaceSize / cell) per axis
+    uint32_t cells;           // gx*gy*gz, cells of one scene
+    uint32_t scenes;          // independent scenes (batched mode)
+    uint32_t dead_key;        // scenes*cells: sort key of particles off the grid
+    float nbr_d2_max;         // largest float d2 with sqrtf(d2) <= cell (host-computed)
+};
+
+// Per-slot neighbour count word: low 31 bits = non-self neighbours stored,
+// bit 31 = particle is off the grid ("dead", UGNSCudaImpl.cu:122-123).
+constexpr uint32_t kDeadBit = 0x80000000u;
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+
+__device__ __forceinline__ float3 xyz(float4 a) { return make_float3(a.x, a.y, a.z); }
+__device__ __forceinline__ float3 add3(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 sub3(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 mul3(float3 a, float s) { return make_float3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 fma3(float3 a, float s, float3 c) {
+    return make_float3(fmaf(a.x, s, c.x), fmaf(a.y, s, c.y), fmaf(a.z, s, c.z));
+}
+
+// ---- decision functions (must match the oracle bit for bit) ---------------
+// Cell coordinate along one axis: floor((x - lb) / cell) with IEEE-rounded
+// subtraction and division (UGNSCudaImpl.cu:28-47).
+__device__ __forceinline__ int cell_coord(float x, float lb, float cell) {
+    return static_cast<int>(floorf(__fdiv_rn(__fsub_rn(x, lb), cell)));
+}
+
+// Sort key of a position: scene*cells + z*gy*gx + y*gx + x (UGNSCudaImpl.cu:58-63),
+// or g.dead_key when the cell lies outside [0, grid) on any axis.  The
+// reference accepts cp == gridSize (an inclusive bound, UGNSCudaImpl.cu:79-80)
+// which aliases into the next row; here that cell is rejected as well.
+__device__ __forceinline__ uint32_t cell_key(float3 p, uint32_t scene, const Grid &g) {
+    int cx = cell_coord(p.x, g.lb_x, g.cell);
+    int cy = cell_coord(p.y, g.lb_y, g.cell);
+    int cz = cell_coord(p.z, g.lb_z, g.cell);
+    bool in = cx >= 0 && cy >= 0 && cz >= 0 && cx < g.gx && cy < g.gy && cz < g.gz;
+    if (!in) return g.dead_key;
+    return scene * g.cells + (uint32_t)cz * (uint32_t)(g.gx * g.gy) + (uint32_t)cy * (uint32_t)g.gx + (uint32_t)cx;
+}
+
+// Squared distance with every operation rounded separately (no contraction):
+// ((dx*dx + dy*dy) + dz*dz), the evaluation order of dot() in the reference.
+__device__ __forceinline__ float dist2_exact(float3 a, float3 b) {
+    float dx = __fsub_rn(a.x, b.x), dy = __fsub_rn(a.y, b.y), dz = __fsub_rn(a.z, b.z);
+    return __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+}
+
+// Neighbour test of UGNSCudaImpl.cu:194: 0 < sqrtf(d2) <= cell.  sqrtf is
+// correctly rounded and monotone, so {d2 : sqrtf(d2) <= cell} = [0, d2_max]
+// where d2_max is found once on the host; sqrtf(d2) > 0 <=> d2 > 0.
+__device__ __forceinline__ bool is_neighbor(float d2, float d2_max) {
+    return d2 > 0.0f && d2 <= d2_max;
+}
+
+// ---- cubic spline kernel (KernelFunctions.cuh:13-56) ----------------------
+// sigma = 8 / (pi h^3) with the reference's pi literal 3.14159265f.
+struct Cubic {
+    float h, inv_h, sigma;      // sigma = 8/(pi h^3)
+    float grad_scale;           // sigma / h
+    float w0;                   // W(0) = sigma
+};
+
+__host__ inline Cubic make_cubic(float h) {
+    Cubic c;
+    c.h = h;
+    c.inv_h = 1.0f / h;
+    const float pi = 3.14159265f;
+    c.sigma = 8.0f / pi / (h * h * h);
+    c.grad_scale = c.sigma / h;
+    c.w0 = c.sigma;
+    return c;
+}
+
+// W(r) from r^2.  q <= 0.5: sigma(6q^3 - 6q^2 + 1); q <= 1: 2 sigma (1-q)^3.
+__device__ __forceinline__ float cubic_w(float r2, const Cubic &k) {
+    float r = __fsqrt_rn(r2);
+    float q = r * k.inv_h;
+    float t = 1.0f - q;
+    float w = (q <= 0.5f) ? fmaf(6.0f * q * q, q - 1.0f, 1.0f) : 2.0f * t * t * t;
+    return q <= 1.0f ? k.sigma * w : 0.0f;
+}
+
+// Scalar factor f such that gradW(r_vec) = f * r_vec:
+// f = sigma/(r h) * (q <= 0.5 ? 6(3q^2 - 2q) : -6(1-q)^2), zero for q < 1e-6 or q > 1.
+__device__ __forceinline__ float cubic_grad_factor(float r2, const Cubic &k) {
+    float rinv = __frsqrt_rn(r2);
+    float r = r2 * rinv;
+    float q = r * k.inv_h;
+    float t = 1.0f - q;
+    float g = (q <= 0.5f) ? 6.0f * q * (3.0f * q - 2.0f) : -6.0f * t * t;
+    float f = k.grad_scale * g * rinv;
+    return (q >= 1e-6f && q <= 1.0f) ? f : 0.0f;  // NaN (r2 == 0) -> 0
+}
+
+// W and the gradient factor together (one sqrt).
+__device__ __forceinline__ void cubic_w_grad(float r2, const Cubic &k, float &w, float &f) {
+    float rinv = __frsqrt_rn(r2);
+    float r = r2 * rinv;
+    float q = r * k.inv_h;
+    float t = 1.0f - q;
+    bool inner = q <= 0.5f;
+    float wi = fmaf(6.0f * q * q, q - 1.0f, 1.0f);
+    float wo = 2.0f * t * t * t;
+    w = q <= 1.0f ? k.sigma * (inner ? wi : wo) : 0.0f;
+    float g = inner ? 6.0f * q * (3.0f * q - 2.0f) : -6.0f * t * t;
+    f = (q >= 1e-6f && q <= 1.0f) ? k.grad_scale * g * rinv : 0.0f;
+}
+
+// Akinci cohesion spline C(r) (KernelFunctions.cuh:58-69), c = 32/(pi h^9).
+__device__ __forceinline__ float surface_tension_c(float r, float h, float c_sigma, float h6_64) {
+    float hr = h - r;
+    float v = hr * hr * hr * r * r * r;
+    if (r * 2.0f > h && r <= h) return c_sigma * v;
+    if (r > 0.0f && r * 2.0f <= h) return c_sigma * (2.0f * v - h6_64);
+    return 0.0f;
+}
+
+// ---- wavefront reductions (wave64) ----------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+}  // namespace vtp

@@ -1,0 +1,256 @@
+// vtp_internal.h - host-side types shared by the HIP translation units:
+// device buffers, stage timing, the uniform-grid search object and the
+// solver context behind the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/vtp.h"
+#include "vtp_device.h"
+
+namespace vtp {
+
+struct HipError : std::runtime_error {
+    hipError_t code;
+    HipError(hipError_t c, const std::string &what) : std::runtime_error(what), code(c) {}
+};
+
+#define VTP_HIP(call)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            throw ::vtp::HipError(e_, std::string(#call) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+// Owning device allocation.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw HipError(e, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        }
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// hipEvent pairs around stages, resolved lazily.  Disabled unless the
+// "stage_timing" option is set; never active while a graph is captured.
+struct StageTimer {
+    bool enabled = false;
+    struct Pending { std::string name; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, double> ms;
+    std::map<std::string, uint64_t> launches;
+    std::vector<std::string> order;
+
+    hipEvent_t get() {
+        if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+        hipEvent_t e;
+        VTP_HIP(hipEventCreate(&e));
+        return e;
+    }
+    void begin(hipStream_t s, const char *name) {
+        if (!enabled) return;
+        Pending p{name, get(), nullptr};
+        VTP_HIP(hipEventRecord(p.a, s));
+        pending.push_back(p);
+    }
+    void end(hipStream_t s) {
+        if (!enabled || pending.empty()) return;
+        pending.back().b = get();
+        VTP_HIP(hipEventRecord(pending.back().b, s));
+    }
+    void collect() {
+        for (auto &p : pending) {
+            if (!p.b) continue;
+            VTP_HIP(hipEventSynchronize(p.b));
+            float t = 0;
+            VTP_HIP(hipEventElapsedTime(&t, p.a, p.b));
+            if (!ms.count(p.name)) order.push_back(p.name);
+            ms[p.name] += t;
+            launches[p.name] += 1;
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    void reset() { collect(); ms.clear(); launches.clear(); order.clear(); }
+    ~StageTimer() {
+        for (auto &p : pending) { (void)hipEventDestroy(p.a); if (p.b) (void)hipEventDestroy(p.b); }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+// Uniform-grid neighbour search state (UniformGridNeighborSearch.hpp:28-63).
+struct Ugns {
+    Grid g{};
+    uint32_t n = 0, M = 0, cap = 0, stride = 0;
+    uint32_t key_bits = 0, passes = 0, tiles = 0;
+    int sorted = 0;                 // which key/val buffer holds the sorted result
+    DevBuf<uint32_t> key[2], val[2], spine, dtot;
+    DevBuf<uint32_t> start, end;    // [dead_key + 1], inclusive end, kEmpty = empty
+    DevBuf<uint32_t> so, perm, orig_new;
+    DevBuf<uint32_t> nbr, cnt;      // ELL lists nbr[t*stride + slot], count word per slot
+
+    void allocate(uint32_t n, const Grid &g, uint32_t max_neighbors);
+    void sort_and_rank(const uint32_t *orig_old, hipStream_t s, StageTimer &tm);
+    void build_neighbors(const float4 *x, hipStream_t s, StageTimer &tm);
+    void export_reference(const uint32_t *orig, uint32_t *d_lists, uint32_t *d_num, hipStream_t s);
+    const uint32_t *sorted_key() const { return key[sorted].p; }
+};
+
+void launch_fill_u32(uint32_t *a, uint32_t n, uint32_t v, hipStream_t s);
+
+// Indices into the device status word array.
+enum StatusWord : uint32_t {
+    kStDead = 0,         // particles off the grid in the last search
+    kStCrash = 1,        // VTP_CRASH_* bits
+    kStDivIter = 2,      // iterations of the last divergence solve
+    kStIncIter = 3,      // iterations of the last density solve
+    kStPhaseIter = 4,    // iterations of the last negative-fraction loop
+    kStDone = 5,         // solver-loop convergence flag
+    kStIter = 6,         // running iteration counter of the active loop
+    kStNegCount = 7,     // newly flagged negative fractions (IMM)
+    kStWords = 16
+};
+
+// Device-side scalars of the iterative solvers (written by kernels).
+struct SolverScalars {
+    double obj_sum[VTP_MAX_OBJECTS];   // per-object residual sums of the current iteration
+    double div_residual;
+    double inc_residual;
+    float dt;                          // current step size (CFL may change it)
+    float vmax;                        // max fluid speed (CFL)
+    uint32_t vmax_bits;
+    uint32_t pad;
+};
+
+struct Object {
+    uint32_t start, end, scene;
+    int32_t mat;
+};
+
+struct Ctx;
+void pbf_initialize(Ctx &c);
+void pbf_step(Ctx &c);
+void pbf_capture_reset(Ctx &c);
+void dfsph_initialize(Ctx &c);
+void dfsph_step(Ctx &c);
+void imm_initialize(Ctx &c);
+void imm_step(Ctx &c);
+void autotune(Ctx &c);
+void solver_tick(Ctx &c);
+
+// Solver context: everything one vtp_ctx owns.
+struct Ctx {
+    vtp_params p{};
+    std::string err;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    bool initialized = false;
+    bool crashed = false;
+    uint64_t steps = 0;
+    float sim_time = 0;
+    uint64_t total_div = 0, total_inc = 0;
+
+    // host staging in attach order
+    std::vector<float> h_pos, h_vel, h_frac;
+    std::vector<int32_t> h_mat;
+    std::vector<uint32_t> h_scene;
+    std::vector<Object> objects;
+    uint32_t n = 0, scenes = 1;
+
+    // derived constants
+    float h = 0, V0 = 0, inv_dt = 0, inv_dt2 = 0;
+    Cubic kern{};
+    Grid grid{};
+
+    // options
+    bool opt_graph = false;
+    bool opt_autotune = false;
+    uint32_t opt_block = 256;
+    uint32_t opt_chunk = 0;   // 0 = adaptive
+    StageTimer timer;
+
+    // attach-order tables (static)
+    DevBuf<int32_t> mat_attach;
+    DevBuf<uint32_t> scene_attach;
+    // slot-order state
+    DevBuf<int32_t> mat;        // material per slot (refreshed from mat_attach)
+    DevBuf<uint32_t> scene;     // scene per slot
+    DevBuf<uint32_t> orig;      // attach id per slot
+    DevBuf<float4> X[3];        // positions (w = mass), X[0] = state
+    DevBuf<float4> V[3];        // velocities (w = volume), V[0] = state
+    Ugns ugns;
+    DevBuf<uint32_t> status;    // StatusWord
+    DevBuf<SolverScalars> scal;
+    uint32_t *h_status = nullptr;   // pinned mirror
+    SolverScalars *h_scal = nullptr;
+
+    // PBF scratch (slot order)
+    DevBuf<float> lam, dens, err_c;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    // PBF adjoint state
+    struct Adjoint;
+    Adjoint *adj = nullptr;
+
+    // DFSPH / IMM fields (slot order)
+    DevBuf<float> rest_density, vis, cr, alpha, delta, kappa;
+    DevBuf<float> rest_attach, vis_attach;   // static per-particle properties, attach order
+    DevBuf<uint32_t> obj_cnt;                // particles per attach object
+    DevBuf<int32_t> obj_fluid;               // 1 if the object is fluid
+    uint32_t iter_hint_div = 1, iter_hint_inc = 1;
+    DevBuf<float4> vadv, acc, normal;
+    // IMM (phase-major: [k*stride + slot])
+    uint32_t P = 0;
+    DevBuf<float> frac[2], frac_in, frac_out;
+    DevBuf<float4> uphase[2], aphase, drift;
+    DevBuf<uint32_t> flag;
+    DevBuf<float> phase_rho, phase_vis;
+    DevBuf<float4> phase_color;
+    DevBuf<float4> color;
+    DevBuf<uint32_t> obj_of;   // object index per slot
+    DevBuf<uint32_t> obj_attach;
+    DevBuf<float> cd, blocking;
+    DevBuf<float> Q[2];          // 9 per particle, [c*stride + slot]
+    DevBuf<float> Lgrad, stress; // 9 per particle
+    DevBuf<float> vis_ct, rheo_beta, rheo_kappa;
+
+    // per-iteration launch shapes picked by the autotuner
+    std::map<std::string, uint32_t> block_of;
+    uint32_t block(const char *k) const {
+        auto it = block_of.find(k);
+        return it == block_of.end() ? opt_block : it->second;
+    }
+
+    ~Ctx();
+};
+
+uint32_t div_up(uint32_t a, uint32_t b);
+void read_status(Ctx &c);
+
+}  // namespace vtp
