@@ -7,8 +7,12 @@ constraint iterations, XSPH 0.01), inputs resident in HBM, the tick replayed
 as a hipGraph.  value = fluid particle-steps per second.
 
 Also reported: the dominant kernel's achieved algorithmic bandwidth against
-the 8 TB/s HBM peak (hipEvents on the context's stream), and the FP64 oracle
-timed on the host cores over a bounded sample of the same workload.
+the 8 TB/s HBM peak (hipEvents on the context's stream), the FP64 oracle
+timed on the host cores over a bounded sample of the same workload, and the
+other parts of the metric: DFSPH at 1M fluid particles, the batched
+64-scene gain (one context holding all scenes vs. one context per scene),
+the adjoint cost ((forward + backward) / forward of the checkpointed
+100-tick rollout), and the three-phase IMM-CT mixture at 1M fluid particles.
 """
 import argparse
 import json
@@ -51,6 +55,119 @@ def build_ctx(vtp, wl, graph=True):
     return ctx, params
 
 
+def timed_steps(torch, ctx, steps):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def bench_dfsph(torch, vtp, scenes, steps, warmup):
+    wl = scenes.dfsph_1m(seed=0)
+    ctx, params = build_ctx(vtp, wl, graph=False)
+    ctx.step(warmup)
+    st0 = ctx.stats()
+    el = timed_steps(torch, ctx, steps)
+    st = ctx.stats()
+    ctx.close()
+    it_div = (st["total_div_iterations"] - st0["total_div_iterations"]) / steps
+    it_inc = (st["total_incomp_iterations"] - st0["total_incomp_iterations"]) / steps
+    return {"workload": wl.name, "fluid_particles": wl.n_fluid, "boundary_particles": wl.n - wl.n_fluid,
+            "value": round(wl.n_fluid * steps / el, 1), "unit": "particle-steps/s",
+            "ms_per_step": round(el * 1e3 / steps, 4), "mean_div_iterations": it_div,
+            "mean_incomp_iterations": it_inc, "steps": steps, "warmup": warmup,
+            "dead_particles": st["dead_particles"]}
+
+
+def bench_immct(torch, vtp, scenes, steps, warmup):
+    wl = scenes.cfg2_immct(seed=0)
+    # the reference's polymer-stress term has no neighbour mass and blows up at
+    # r = 0.005 within two ticks; the timed run uses the mass-weighted form
+    wl.config["IMMCT"]["Extensions"]["massWeighted"] = True
+    ctx, params = build_ctx(vtp, wl, graph=False)
+    ctx.step(warmup)
+    el = timed_steps(torch, ctx, steps)
+    st = ctx.stats()
+    ctx.close()
+    return {"workload": wl.name, "fluid_particles": wl.n_fluid, "boundary_particles": wl.n - wl.n_fluid,
+            "phases": 3, "value": round(wl.n_fluid * steps / el, 1), "unit": "particle-steps/s",
+            "ms_per_step": round(el * 1e3 / steps, 4), "steps": steps, "warmup": warmup,
+            "mass_weighted_stress": True, "last_iterations": [st["div_iterations"], st["incomp_iterations"],
+                                                              st["phase_iterations"]]}
+
+
+def bench_batched(torch, vtp, scenes, steps, warmup):
+    wl = scenes.cfg3_batched(seed=0)
+    ctx, params = build_ctx(vtp, wl, graph=True)
+    ctx.step(warmup)
+    t_batched = timed_steps(torch, ctx, steps)
+    ctx.close()
+    # the same scenes, one context (one launch sequence) per scene
+    per = {}
+    for o in wl.objects:
+        per.setdefault(o.scene, []).append(o)
+    ctxs = []
+    for sc, objs in sorted(per.items()):
+        c = vtp.Context(params)
+        for o in objs:
+            c.add_particles(o.pos, o.mat, o.vel, None, 0)
+        c.set_option("graph", 1)
+        c.initialize()
+        c.step(warmup)
+        ctxs.append(c)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for c in ctxs:
+            c.step(1)
+    torch.cuda.synchronize()
+    t_single = time.perf_counter() - t0
+    for c in ctxs:
+        c.close()
+    return {"workload": wl.name, "scenes": len(per), "fluid_particles": wl.n_fluid,
+            "boundary_particles": wl.n - wl.n_fluid, "steps": steps,
+            "batched_ms_per_step": round(t_batched * 1e3 / steps, 4),
+            "one_context_per_scene_ms_per_step": round(t_single * 1e3 / steps, 4),
+            "gain": round(t_single / t_batched, 3),
+            "batched_value": round(wl.n_fluid * steps / t_batched, 1), "unit": "particle-steps/s"}
+
+
+def bench_adjoint(torch, vtp, scenes, nsteps=100, ckpt=10):
+    wl = scenes.cfg4_adjoint(seed=0)
+    ctx, params = build_ctx(vtp, wl, graph=True)
+    x0, v0 = ctx.read("pos"), ctx.read("vel")
+    ctx.rollout_fwd(10, ckpt)          # warm-up (allocations, graph capture)
+    ctx.write("pos", x0)
+    ctx.write("vel", v0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.rollout_fwd(nsteps, ckpt)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    x = ctx.read("pos")
+    nf = wl.n_fluid
+    import numpy as np
+    gx = np.zeros_like(x)
+    target = np.array([0.7, 0.2, 0.5], np.float32)
+    gx[:nf] = 2 * (x[:nf].mean(0) - target) / nf
+    loss_d, gl = ctx.density_loss()
+    ctx.rollout_bwd(gx + gl, None)      # warm-up of the tape allocation
+    ctx.write("pos", x0)
+    ctx.write("vel", v0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.rollout_fwd(nsteps, ckpt)
+    _, gl = ctx.density_loss()
+    gv0, _, gk = ctx.rollout_bwd(gx + gl, None)
+    torch.cuda.synchronize()
+    t_both = time.perf_counter() - t0
+    ctx.close()
+    return {"workload": wl.name, "fluid_particles": nf, "steps": nsteps, "checkpoint_every": ckpt,
+            "forward_ms": round(t_fwd * 1e3, 3), "forward_backward_ms": round(t_both * 1e3, 3),
+            "cost_ratio": round(t_both / t_fwd, 3), "grad_xsph_k": gk}
+
+
 def cpu_baseline(wl, params, budget_s=20.0):
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle
@@ -79,6 +196,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--only-headline", action="store_true", help="skip the DFSPH / batched / adjoint / IMM-CT legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1 "
@@ -127,6 +245,18 @@ def main():
     st = ctx.stats()
     ctx.close()
 
+    extra = {}
+    if not args.only_headline:
+        for name, fn in (("dfsph_1m", lambda: bench_dfsph(torch, vtp, scenes, 20, 5)),
+                         ("batched_64", lambda: bench_batched(torch, vtp, scenes, 10, 3)),
+                         ("adjoint", lambda: bench_adjoint(torch, vtp, scenes)),
+                         ("immct_1m", lambda: bench_immct(torch, vtp, scenes, 5, 2))):
+            try:
+                extra[name] = fn()
+            except Exception as e:   # a failing leg is reported, not hidden
+                extra[name] = {"error": f"{type(e).__name__}: {e}"}
+            print(f"bench: {name} done", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl, params, args.cpu_budget)
@@ -145,6 +275,10 @@ def main():
         "stages_ms_per_step": {k: round(v[0] / prof_steps, 5) for k, v in stages.items()},
         "stage_total_ms_per_step": round(total / prof_steps, 4),
         "dead_particles": st["dead_particles"],
+        "dfsph_1m": extra.get("dfsph_1m"),
+        "batched_64": extra.get("batched_64"),
+        "adjoint": extra.get("adjoint"),
+        "immct_1m": extra.get("immct_1m"),
     }
     print(json.dumps(line))
 

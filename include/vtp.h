@@ -114,6 +114,9 @@ typedef struct vtp_params {
     float cfl_number;
     float dt_max;
     uint32_t max_solver_iterations;  /* iteration cap, reference: 100             */
+    int32_t mass_weighted;           /* 1 = neighbour mass m_j in the PBF constraint
+                                        gradient and the IMM-CT polymer-stress term
+                                        (the reference omits it); default 0       */
 } vtp_params;
 
 /* Readable / writable per-particle fields (attach order). */

@@ -28,6 +28,7 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <map>
 #include <vector>
 
 #ifdef _OPENMP
@@ -116,6 +117,7 @@ struct Sim {
     // constants (float values of the reference promoted to double)
     float hf = 0, cellf = 0;
     double h = 0, V0 = 0, dt = 0;
+    double xsph = 0;   // XSPH coefficient in double (learnable in the adjoint tests)
     Kern k;
     // grid
     int gx = 0, gy = 0, gz = 0;
@@ -299,6 +301,7 @@ void pbf_tick(Sim &s) {
             double l = 0;
             for_nbrs(s, i, [&](uint32_t j) {
                 V3 w = s.k.grad(s.pos[i] - s.pos[j]);
+                if (s.p.mass_weighted) w = w * s.mass[j];   // extension: m_j in the gradient
                 V3 egj = w * (-1.0 / rho0);
                 eg += w;
                 l += dot(egj, egj);
@@ -310,7 +313,10 @@ void pbf_tick(Sim &s) {
         par_for(s.n, [&](uint32_t i) {  // compute_dx_cuda :124-144
             if (s.mat[i] != VTP_MAT_FLUID) return;
             V3 d{};
-            for_nbrs(s, i, [&](uint32_t j) { d += (s.lamb[i] + s.lamb[j]) * s.k.grad(s.pos[i] - s.pos[j]); });
+            for_nbrs(s, i, [&](uint32_t j) {
+                double mw = s.p.mass_weighted ? s.mass[j] : 1.0;
+                d += (mw * (s.lamb[i] + s.lamb[j])) * s.k.grad(s.pos[i] - s.pos[j]);
+            });
             s.dxv[i] = d * (1.0 / rho0);
         });
         par_for(s.n, [&](uint32_t i) {  // apply_dx_cuda :159-170
@@ -323,7 +329,7 @@ void pbf_tick(Sim &s) {
         if (s.mat[i] != VTP_MAT_FLUID) return;
         V3 dv{};
         for_nbrs(s, i, [&](uint32_t j) {
-            dv += (s.p.xsph_k * s.volume[j] * s.k.W(s.pos[i] - s.pos[j])) * (s.vel[j] - s.vel[i]);
+            dv += (s.xsph * s.volume[j] * s.k.W(s.pos[i] - s.pos[j])) * (s.vel[j] - s.vel[i]);
         });
         s.dxv[i] = dv * dt;
     });
@@ -569,6 +575,7 @@ extern "C" orc_sim *orc_create(const vtp_params *p, uint32_t n, const float *pos
     s.h = s.hf;
     s.V0 = std::pow(2.0f * p->particle_radius, 3.0f);
     s.dt = p->time_step;
+    s.xsph = p->xsph_k;
     s.k = Kern(s.h);
     s.gx = (int)(uint32_t)std::ceil(p->sim_space_size[0] / s.cellf);
     s.gy = (int)(uint32_t)std::ceil(p->sim_space_size[1] / s.cellf);

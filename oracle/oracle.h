@@ -53,12 +53,13 @@ int orc_neighbor_test(const float *a, const float *b, float cell);
 void orc_set_threads(int n);
 int orc_get_threads(void);
 
-/* PBF adjoint in FP64: forward nsteps from the current state with the
- * current XSPH coefficient, then reverse.  grad_x/grad_v: dL/d(final x,v)
- * (attach order, 3 per particle); outputs dL/dv0, dL/dx0 (may be NULL) and
- * dL/dk. */
-int orc_pbf_adjoint(orc_sim *s, uint32_t nsteps, const double *grad_x, const double *grad_v, double *grad_v0,
-                    double *grad_x0, double *grad_k);
+/* PBF adjoint in FP64 (same split as the C ABI): a taped forward of nsteps
+ * ticks from the current state, the density loss at the current state, and
+ * the reverse pass.  grad_x/grad_v: dL/d(final x, v) in attach order (3 per
+ * particle, may be NULL); outputs dL/dv0, dL/dx0 (may be NULL) and dL/dk. */
+int orc_pbf_rollout_fwd(orc_sim *s, uint32_t nsteps);
+int orc_pbf_rollout_bwd(orc_sim *s, const double *grad_x, const double *grad_v, double *grad_v0,
+                        double *grad_x0, double *grad_k);
 /* Mean squared density deviation over the last neighbour lists and dL/dx. */
 int orc_pbf_density_loss(orc_sim *s, double *loss, double *grad_x);
 int orc_pbf_set_xsph(orc_sim *s, double k);

@@ -39,7 +39,8 @@ def load():
         lib.orc_set_threads.argtypes = [C.c_int]
         lib.orc_set_threads.restype = None
         lib.orc_get_threads.restype = C.c_int
-        lib.orc_pbf_adjoint.argtypes = [C.c_void_p, C.c_uint32, _D, _D, _D, _D, _D]
+        lib.orc_pbf_rollout_fwd.argtypes = [C.c_void_p, C.c_uint32]
+        lib.orc_pbf_rollout_bwd.argtypes = [C.c_void_p, _D, _D, _D, _D, _D]
         lib.orc_pbf_density_loss.argtypes = [C.c_void_p, _D, _D]
         lib.orc_pbf_set_xsph.argtypes = [C.c_void_p, C.c_double]
         lib.orc_kernel.argtypes = [C.c_double, _D, _D, _D, _D]
@@ -137,15 +138,20 @@ class Oracle:
         return {"div_iterations": a.value, "incomp_iterations": b.value, "phase_iterations": c.value,
                 "div_residual": d1.value, "incomp_residual": d2.value, "crash_flags": e.value}
 
-    def adjoint(self, nsteps, grad_x=None, grad_v=None):
+    def rollout_fwd(self, nsteps):
+        rc = self.lib.orc_pbf_rollout_fwd(self.h, nsteps)
+        if rc != 0:
+            raise RuntimeError(f"orc_pbf_rollout_fwd failed: {rc}")
+
+    def rollout_bwd(self, grad_x=None, grad_v=None):
         gx = None if grad_x is None else np.ascontiguousarray(grad_x, np.float64)
         gv = None if grad_v is None else np.ascontiguousarray(grad_v, np.float64)
         gv0 = np.zeros((self.n, 3))
         gx0 = np.zeros((self.n, 3))
         gk = np.zeros(1)
-        rc = self.lib.orc_pbf_adjoint(self.h, nsteps, _dp(gx), _dp(gv), _dp(gv0), _dp(gx0), _dp(gk))
+        rc = self.lib.orc_pbf_rollout_bwd(self.h, _dp(gx), _dp(gv), _dp(gv0), _dp(gx0), _dp(gk))
         if rc != 0:
-            raise RuntimeError(f"orc_pbf_adjoint failed: {rc}")
+            raise RuntimeError(f"orc_pbf_rollout_bwd failed: {rc}")
         return gv0, gx0, float(gk[0])
 
     def density_loss(self):

@@ -1,9 +1,566 @@
-// adjoint.hip - placeholder until the PBF adjoint lands.
+// adjoint.hip - reverse mode of the PBF tick on the GPU (no reference counterpart).
+//
+// Rollout forward: ticks run as usual (graph replay allowed) and the full
+// state (positions+mass, velocities+volume, slot->attach map) is saved every
+// `ckpt_every` ticks.  Rollout backward: segments are processed last to
+// first; each is recomputed from its checkpoint with a taped forward (lists,
+// slot permutation, positions entering every constraint iteration, lambda,
+// C, sum_j gradW and S, velocities entering XSPH) and then reversed tick by
+// tick:
+//   XSPH       x' = x + dt dv, v' = v + dv, dv_i = k sum_j V_j (v_j - v_i) W_ij
+//   iteration  x' = x + dx, v' = v + dx/dt, dx_i = (1/rho0) sum_j (l_i + l_j) gradW_ij,
+//              l_i = -C_i/(S_i + 1e-6), C_i = max(rho_i, rho0)/rho0 - 1,
+//              S_i = (1/rho0^2)(sum_j |gradW_ij|^2 + |sum_j gradW_ij|^2)
+//   reorder    x_new[i] = x_old[perm[i]]  ->  xbar_old[perm[i]] = xbar_new[i]
+//   predict    v1 = v0 + g dt, x1 = x0 + v1 dt
+// with neighbour lists held fixed, H_ij = f I + (f'(r)/r) r r^T the Hessian
+// of W, and symmetric lists (j in N(i) <=> i in N(j)), which holds whenever
+// no list is cut by maxNeighborNum.
+#include <cstring>
+#include <memory>
+
 #include "vtp_internal.h"
+
 namespace vtp {
-void autotune(Ctx &) {}
+
+void pbf_step(Ctx &c);
+
+// Hessian-vector product H(r) a of the cubic spline, with gradW = f r.
+__device__ __forceinline__ void kernel_f_d(float r2, const Cubic &k, float &f, float &d) {
+    float rinv = __frsqrt_rn(r2);
+    float r = r2 * rinv;
+    float q = r * k.inv_h;
+    bool inner = q <= 0.5f;
+    float t = 1.0f - q;
+    float g = inner ? 6.0f * q * fmaf(3.0f, q, -2.0f) : -6.0f * t * t;
+    float gp = inner ? 6.0f * fmaf(6.0f, q, -2.0f) : 12.0f * t;
+    bool ok = q >= 1e-6f && q <= 1.0f;
+    f = ok ? k.grad_scale * g * rinv : 0.0f;
+    d = ok ? k.grad_scale * rinv * rinv * (gp * k.inv_h - g * rinv) : 0.0f;
+}
+
+__global__ void k_adj_mat_from_orig(const uint32_t *__restrict__ orig, const int32_t *__restrict__ mat_attach,
+                                    const uint32_t *__restrict__ scene_attach, uint32_t n, int32_t *__restrict__ mat,
+                                    uint32_t *__restrict__ scene) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t a = orig[i];
+    mat[i] = mat_attach[a];
+    if (scene) scene[i] = scene_attach[a];
+}
+
+// u = xbar dt + vbar (adjoint of dv), zero for non-fluid.
+__global__ void k_adj_u(const float4 *__restrict__ xb, const float4 *__restrict__ vb,
+                        const int32_t *__restrict__ mat, uint32_t n, float dt, float4 *__restrict__ u) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 a = make_float4(0, 0, 0, 0);
+    if (mat[i] == VTP_MAT_FLUID) {
+        float4 x = xb[i], v = vb[i];
+        a = make_float4(x.x * dt + v.x, x.y * dt + v.y, x.z * dt + v.z, 0.0f);
+    }
+    u[i] = a;
+}
+
+// w = xbar + vbar/dt (adjoint of dx of a constraint iteration), zero for non-fluid.
+__global__ void k_adj_w(const float4 *__restrict__ xb, const float4 *__restrict__ vb,
+                        const int32_t *__restrict__ mat, uint32_t n, float inv_dt, float4 *__restrict__ w) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 a = make_float4(0, 0, 0, 0);
+    if (mat[i] == VTP_MAT_FLUID) {
+        float4 x = xb[i], v = vb[i];
+        a = make_float4(x.x + v.x * inv_dt, x.y + v.y * inv_dt, x.z + v.z * inv_dt, 0.0f);
+    }
+    w[i] = a;
+}
+
+// Transposed XSPH Jacobian:
+//   xbar_p += k sum_j [V_j u_p.(v_j - v_p) + V_p u_j.(v_p - v_j)] gradW_pj
+//   vbar_p += k V_p sum_j W_pj u_j - k u_p sum_j V_j W_pj
+//   kbar   += sum_p u_p . sum_j V_j (v_j - v_p) W_pj
+__global__ __launch_bounds__(256) void k_adj_xsph(const float4 *__restrict__ X, const float4 *__restrict__ V,
+                                                  const float4 *__restrict__ U, const int32_t *__restrict__ mat,
+                                                  const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ cnt,
+                                                  uint32_t n, uint32_t stride, PbfK q, float4 *__restrict__ xb,
+                                                  float4 *__restrict__ vb, double *kbar) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    double kp = 0.0;
+    if (p < n && mat[p] == VTP_MAT_FLUID) {
+        const float4 xp = X[p], vp = V[p], up = U[p];
+        const float k = q.xsph_k;
+        const uint32_t nn = cnt[p] & ~kDeadBit;
+        float ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0, sw = 0, kk = 0;
+        struct D { float4 x, v, u; };
+        for_each_neighbor(
+            nbr, stride, p, nn, [&](uint32_t j) { return D{X[j], V[j], U[j]}; },
+            [&](const D &e) {
+                float dx = xp.x - e.x.x, dy = xp.y - e.x.y, dz = xp.z - e.x.z;
+                float W, f;
+                cubic_w_grad(dx * dx + dy * dy + dz * dz, q.k, W, f);
+                float dvx = e.v.x - vp.x, dvy = e.v.y - vp.y, dvz = e.v.z - vp.z;
+                float upd = up.x * dvx + up.y * dvy + up.z * dvz;
+                float ujd = -(e.u.x * dvx + e.u.y * dvy + e.u.z * dvz);
+                float s = k * (e.v.w * upd + vp.w * ujd) * f;
+                ax = fmaf(dx, s, ax);
+                ay = fmaf(dy, s, ay);
+                az = fmaf(dz, s, az);
+                float c = k * vp.w * W;
+                bx = fmaf(c, e.u.x, bx);
+                by = fmaf(c, e.u.y, by);
+                bz = fmaf(c, e.u.z, bz);
+                sw = fmaf(e.v.w, W, sw);
+                kk = fmaf(e.v.w * W, upd, kk);
+            });
+        float4 a = xb[p], b = vb[p];
+        a.x += ax;
+        a.y += ay;
+        a.z += az;
+        b.x += bx - k * sw * up.x;
+        b.y += by - k * sw * up.y;
+        b.z += bz - k * sw * up.z;
+        xb[p] = a;
+        vb[p] = b;
+        kp = kk;
+    }
+    double s = wave_sum(kp);
+    if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(kbar, s);
+}
+
+// lambda_bar_i = (1/rho0) sum_j (w_i - w_j) . gradW_ij, then the adjoints of the
+// density (rb, zero where clamped) and of S (sb): rs = (rb, sb).
+__global__ __launch_bounds__(256) void k_adj_lambar(const float4 *__restrict__ X, const float4 *__restrict__ Wv,
+                                                    const float *__restrict__ Cc, const float4 *__restrict__ sg,
+                                                    const int32_t *__restrict__ mat, const uint32_t *__restrict__ nbr,
+                                                    const uint32_t *__restrict__ cnt, uint32_t n, uint32_t stride,
+                                                    PbfK q, float2 *__restrict__ rs) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (mat[i] != VTP_MAT_FLUID) {
+        rs[i] = make_float2(0, 0);
+        return;
+    }
+    const float4 xi = X[i], wi = Wv[i];
+    const uint32_t nn = cnt[i] & ~kDeadBit;
+    float lb = 0.0f;
+    struct D { float4 x, w; };
+    for_each_neighbor(
+        nbr, stride, i, nn, [&](uint32_t j) { return D{X[j], Wv[j]}; },
+        [&](const D &e) {
+            float dx = xi.x - e.x.x, dy = xi.y - e.x.y, dz = xi.z - e.x.z;
+            float f = cubic_grad_factor(dx * dx + dy * dy + dz * dz, q.k);
+            lb = fmaf(((wi.x - e.w.x) * dx + (wi.y - e.w.y) * dy + (wi.z - e.w.z) * dz), f, lb);
+        });
+    lb *= q.inv_rho0;
+    const float C = Cc[i];
+    const float den = sg[i].w + 1e-6f;
+    const float Cb = -lb / den;
+    const float Sb = lb * C / (den * den);
+    const float rb = C > 0.0f ? Cb * q.inv_rho0 : 0.0f;
+    rs[i] = make_float2(rb, Sb);
+}
+
+// Transposed constraint-iteration Jacobian:
+//   xbar_p += sum_j H_pj [ (l_p + l_j)/rho0 (w_p - w_j)
+//                          + 2/rho0^2 (sb_p (gradW_pj + G_p) + sb_j (gradW_pj - G_j)) ]
+//            + gradW_pj (rb_p m_j + rb_j m_p)
+__global__ __launch_bounds__(256) void k_adj_xbar(const float4 *__restrict__ X, const float4 *__restrict__ Wv,
+                                                  const float2 *__restrict__ rs, const float *__restrict__ lam,
+                                                  const float4 *__restrict__ sg, const int32_t *__restrict__ mat,
+                                                  const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ cnt,
+                                                  uint32_t n, uint32_t stride, PbfK q, float4 *__restrict__ xb) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || mat[p] != VTP_MAT_FLUID) return;
+    const float4 xp = X[p], wp = Wv[p], Gp = sg[p];
+    const float2 rp = rs[p];
+    const float lp = lam[p];
+    const float c2 = 2.0f * q.inv_rho0 * q.inv_rho0;
+    const uint32_t nn = cnt[p] & ~kDeadBit;
+    float ax = 0, ay = 0, az = 0;
+    struct D { float4 x, w, G; float2 r; float l; };
+    for_each_neighbor(
+        nbr, stride, p, nn, [&](uint32_t j) { return D{X[j], Wv[j], sg[j], rs[j], lam[j]}; },
+        [&](const D &e) {
+            float rx = xp.x - e.x.x, ry = xp.y - e.x.y, rz = xp.z - e.x.z;
+            float f, d;
+            kernel_f_d(rx * rx + ry * ry + rz * rz, q.k, f, d);
+            float gx = f * rx, gy = f * ry, gz = f * rz;
+            float cl = (lp + e.l) * q.inv_rho0;
+            float tx = cl * (wp.x - e.w.x) + c2 * (rp.y * (gx + Gp.x) + e.r.y * (gx - e.G.x));
+            float ty = cl * (wp.y - e.w.y) + c2 * (rp.y * (gy + Gp.y) + e.r.y * (gy - e.G.y));
+            float tz = cl * (wp.z - e.w.z) + c2 * (rp.y * (gz + Gp.z) + e.r.y * (gz - e.G.z));
+            float rt = d * (rx * tx + ry * ty + rz * tz);
+            float dm = rp.x * e.x.w + e.r.x * xp.w;
+            ax += f * tx + rt * rx + dm * gx;
+            ay += f * ty + rt * ry + dm * gy;
+            az += f * tz + rt * rz + dm * gz;
+        });
+    float4 a = xb[p];
+    a.x += ax;
+    a.y += ay;
+    a.z += az;
+    xb[p] = a;
+}
+
+// Undo the reorder (x_new[i] = x_old[perm[i]]) and the predict (v1 = v0 + g dt,
+// x1 = x0 + v1 dt: vbar0 = vbar1 + dt xbar1, xbar0 = xbar1).
+__global__ void k_adj_unpermute_predict(const uint32_t *__restrict__ perm, const float4 *__restrict__ xb,
+                                        const float4 *__restrict__ vb, uint32_t n, float dt,
+                                        float4 *__restrict__ xo, float4 *__restrict__ vo) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t o = perm[i];
+    float4 x = xb[i], v = vb[i];
+    xo[o] = x;
+    vo[o] = make_float4(v.x + dt * x.x, v.y + dt * x.y, v.z + dt * x.z, 0.0f);
+}
+
+// Density loss: L = mean_f (rho_f/rho0 - 1)^2 over the current lists
+// (unclamped, self included); rb_i = dL/drho_i.
+__global__ void k_dl_rho(const float4 *__restrict__ X, const int32_t *__restrict__ mat,
+                         const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ cnt, uint32_t n,
+                         uint32_t stride, PbfK q, float inv_nf, float *__restrict__ rb, double *loss) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double e2 = 0.0;
+    if (i < n) {
+        rb[i] = 0.0f;
+        if (mat[i] == VTP_MAT_FLUID) {
+            const float4 xi = X[i];
+            const uint32_t w = cnt[i];
+            const uint32_t nn = w & ~kDeadBit;
+            float rho = (w & kDeadBit) ? 0.0f : xi.w * q.k.w0;
+            for_each_neighbor(
+                nbr, stride, i, nn, [&](uint32_t j) { return X[j]; },
+                [&](const float4 &xj) {
+                    float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                    rho = fmaf(xj.w, cubic_w(dx * dx + dy * dy + dz * dz, q.k), rho);
+                });
+            float e = rho * q.inv_rho0 - 1.0f;
+            e2 = (double)e * e;
+            rb[i] = 2.0f * e * q.inv_rho0 * inv_nf;
+        }
+    }
+    double s = wave_sum(e2);
+    if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(loss, s);
+}
+
+__global__ void k_dl_grad(const float4 *__restrict__ X, const float *__restrict__ rb, const int32_t *__restrict__ mat,
+                          const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ cnt, uint32_t n,
+                          uint32_t stride, PbfK q, float4 *__restrict__ g) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float ax = 0, ay = 0, az = 0;
+    if (mat[p] == VTP_MAT_FLUID) {
+        const float4 xp = X[p];
+        const float rp = rb[p];
+        const uint32_t nn = cnt[p] & ~kDeadBit;
+        struct D { float4 x; float r; };
+        for_each_neighbor(
+            nbr, stride, p, nn, [&](uint32_t j) { return D{X[j], rb[j]}; },
+            [&](const D &e) {
+                float dx = xp.x - e.x.x, dy = xp.y - e.x.y, dz = xp.z - e.x.z;
+                float f = cubic_grad_factor(dx * dx + dy * dy + dz * dz, q.k) * (rp * e.x.w + e.r * xp.w);
+                ax = fmaf(dx, f, ax);
+                ay = fmaf(dy, f, ay);
+                az = fmaf(dz, f, az);
+            });
+    }
+    g[p] = make_float4(ax, ay, az, 0.0f);
+}
+
+// ------------------------------------------------------------------ host side
+struct TapeBufs {
+    DevBuf<uint32_t> perm, orig, cnt, nbr;
+    DevBuf<float4> x[17];
+    DevBuf<float> lam[16], C[16];
+    DevBuf<float4> sg[16];
+    DevBuf<float4> vlast;
+    PbfTapeStep ptrs{};
+};
+
+struct Checkpoint {
+    DevBuf<float4> x, v;
+    DevBuf<uint32_t> orig;
+};
+
+struct AdjointState {
+    uint32_t nsteps = 0, every = 0, nseg = 0;
+    std::vector<std::unique_ptr<Checkpoint>> ckpt;
+    std::vector<std::unique_ptr<TapeBufs>> tape;
+    DevBuf<float4> xb, vb, xb2, vb2, U, Wv;
+    DevBuf<float2> rs;
+    DevBuf<double> kbar;
+    DevBuf<int32_t> mat;
+    bool valid = false;
+};
+
+static AdjointState &adj(Ctx &c) {
+    if (!c.adj_state) c.adj_state = std::shared_ptr<void>(new AdjointState(), [](void *p) { delete (AdjointState *)p; });
+    return *static_cast<AdjointState *>(c.adj_state.get());
+}
+
+void launch_mat_from_orig(Ctx &c) {
+    hipLaunchKernelGGL(k_adj_mat_from_orig, dim3(div_up(c.n, 256)), dim3(256), 0, c.stream, c.orig.p,
+                       c.mat_attach.p, c.scene_attach.p, c.n, c.mat.p, c.scene.p);
+}
+
+static void save_ckpt(Ctx &c, Checkpoint &k) {
+    const size_t n = c.n;
+    VTP_HIP(hipMemcpyAsync(k.x.p, c.X[0].p, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
+    VTP_HIP(hipMemcpyAsync(k.v.p, c.V[0].p, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
+    VTP_HIP(hipMemcpyAsync(k.orig.p, c.orig.p, n * 4, hipMemcpyDeviceToDevice, c.stream));
+}
+
+static void load_ckpt(Ctx &c, Checkpoint &k) {
+    const uint32_t n = c.n;
+    VTP_HIP(hipMemcpyAsync(c.X[0].p, k.x.p, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
+    VTP_HIP(hipMemcpyAsync(c.V[0].p, k.v.p, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
+    VTP_HIP(hipMemcpyAsync(c.orig.p, k.orig.p, (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream));
+    hipLaunchKernelGGL(k_adj_mat_from_orig, dim3(div_up(n, 256)), dim3(256), 0, c.stream, c.orig.p,
+                       c.mat_attach.p, c.scene_attach.p, n, c.mat.p, c.scene.p);
+}
+
+static void alloc_tape(Ctx &c, TapeBufs &t) {
+    const uint32_t n = c.n, K = c.p.iteration_num;
+    t.perm.alloc(n);
+    t.orig.alloc(n);
+    t.cnt.alloc(n);
+    t.nbr.alloc((size_t)c.ugns.cap * c.ugns.stride + 1);
+    for (uint32_t m = 0; m <= K; ++m) t.x[m].alloc(n);
+    for (uint32_t m = 0; m < K; ++m) {
+        t.lam[m].alloc(n);
+        t.C[m].alloc(n);
+        t.sg[m].alloc(n);
+    }
+    t.vlast.alloc(n);
+    t.ptrs.perm = t.perm.p;
+    t.ptrs.orig = t.orig.p;
+    t.ptrs.cnt = t.cnt.p;
+    t.ptrs.nbr = t.nbr.p;
+    for (uint32_t m = 0; m <= K; ++m) t.ptrs.x[m] = t.x[m].p;
+    for (uint32_t m = 0; m < K; ++m) {
+        t.ptrs.lam[m] = t.lam[m].p;
+        t.ptrs.C[m] = t.C[m].p;
+        t.ptrs.sg[m] = t.sg[m].p;
+    }
+    t.ptrs.v_last = t.vlast.p;
+}
+
+static void reverse_tick(Ctx &c, AdjointState &A, TapeBufs &t) {
+    const uint32_t n = c.n, K = c.p.iteration_num, nb = div_up(n, 256);
+    const PbfK q = pbf_constants(c);
+    Ugns &u = c.ugns;
+    hipLaunchKernelGGL(k_adj_mat_from_orig, dim3(nb), dim3(256), 0, c.stream, t.orig.p, c.mat_attach.p,
+                       c.scene_attach.p, n, A.mat.p, (uint32_t *)nullptr);
+    c.timer.begin(c.stream, "adj_xsph");
+    hipLaunchKernelGGL(k_adj_u, dim3(nb), dim3(256), 0, c.stream, A.xb.p, A.vb.p, A.mat.p, n, q.dt, A.U.p);
+    hipLaunchKernelGGL(k_adj_xsph, dim3(nb), dim3(256), 0, c.stream, t.x[K].p, t.vlast.p, A.U.p, A.mat.p, t.nbr.p,
+                       t.cnt.p, n, u.stride, q, A.xb.p, A.vb.p, A.kbar.p);
+    c.timer.end(c.stream);
+    for (int m = (int)K - 1; m >= 0; --m) {
+        c.timer.begin(c.stream, "adj_iteration");
+        hipLaunchKernelGGL(k_adj_w, dim3(nb), dim3(256), 0, c.stream, A.xb.p, A.vb.p, A.mat.p, n, q.inv_dt, A.Wv.p);
+        hipLaunchKernelGGL(k_adj_lambar, dim3(nb), dim3(256), 0, c.stream, t.x[m].p, A.Wv.p, t.C[m].p, t.sg[m].p,
+                           A.mat.p, t.nbr.p, t.cnt.p, n, u.stride, q, A.rs.p);
+        hipLaunchKernelGGL(k_adj_xbar, dim3(nb), dim3(256), 0, c.stream, t.x[m].p, A.Wv.p, A.rs.p, t.lam[m].p,
+                           t.sg[m].p, A.mat.p, t.nbr.p, t.cnt.p, n, u.stride, q, A.xb.p);
+        c.timer.end(c.stream);
+    }
+    hipLaunchKernelGGL(k_adj_unpermute_predict, dim3(nb), dim3(256), 0, c.stream, t.perm.p, A.xb.p, A.vb.p, n, q.dt,
+                       A.xb2.p, A.vb2.p);
+    std::swap(A.xb.p, A.xb2.p);
+    std::swap(A.vb.p, A.vb2.p);
+}
+
 }  // namespace vtp
-extern "C" vtp_status vtp_pbf_rollout_fwd(vtp_ctx *, uint32_t, uint32_t) { return VTP_EINVAL; }
-extern "C" vtp_status vtp_pbf_rollout_bwd(vtp_ctx *, const float *, const float *, float *, float *, float *) { return VTP_EINVAL; }
-extern "C" vtp_status vtp_pbf_density_loss(vtp_ctx *, double *, float *) { return VTP_EINVAL; }
-extern "C" vtp_status vtp_pbf_set_xsph(vtp_ctx *, float) { return VTP_EINVAL; }
+
+using namespace vtp;
+
+template <typename F>
+static vtp_status adj_guarded(Ctx &c, F &&f) {
+    try {
+        c.err.clear();
+        return f();
+    } catch (const HipError &e) {
+        c.err = e.what();
+        return e.code == hipErrorOutOfMemory ? VTP_ENOMEM : VTP_EHIP;
+    } catch (const std::exception &e) {
+        c.err = e.what();
+        return VTP_EINVAL;
+    }
+}
+
+static vtp_status adj_check(Ctx &c) {
+    if (c.p.solver != VTP_PBF) { c.err = "rollout gradients exist for PBF only"; return VTP_EINVAL; }
+    if (c.p.mass_weighted) { c.err = "the adjoint differentiates the reference (mass-free) formulation only"; return VTP_EINVAL; }
+    if (c.p.iteration_num > 16) { c.err = "the adjoint supports at most 16 constraint iterations"; return VTP_EINVAL; }
+    if (!c.initialized) { c.err = "not initialized"; return VTP_ESTATE; }
+    return VTP_OK;
+}
+
+extern "C" vtp_status vtp_pbf_rollout_fwd(vtp_ctx *ctx, uint32_t nsteps, uint32_t ckpt_every) {
+    if (!ctx) return VTP_EINVAL;
+    Ctx &c = ctx->c;
+    vtp_status s = adj_check(c);
+    if (s != VTP_OK) return s;
+    if (nsteps == 0 || ckpt_every == 0) { c.err = "nsteps and ckpt_every must be positive"; return VTP_EINVAL; }
+    if (c.crashed) { c.err = "solver is crashed"; return VTP_ECRASH; }
+    return adj_guarded(c, [&]() -> vtp_status {
+        AdjointState &A = adj(c);
+        A.valid = false;
+        A.nsteps = nsteps;
+        A.every = ckpt_every;
+        A.nseg = (nsteps + ckpt_every - 1) / ckpt_every;
+        if (A.ckpt.size() != A.nseg + 1) {
+            A.ckpt.clear();
+            for (uint32_t k = 0; k <= A.nseg; ++k) {
+                auto cp = std::make_unique<Checkpoint>();
+                cp->x.alloc(c.n);
+                cp->v.alloc(c.n);
+                cp->orig.alloc(c.n);
+                A.ckpt.push_back(std::move(cp));
+            }
+        }
+        save_ckpt(c, *A.ckpt[0]);
+        for (uint32_t t = 0; t < nsteps; ++t) {
+            pbf_step(c);
+            c.steps += 1;
+            c.sim_time += c.p.time_step;
+            if ((t + 1) % ckpt_every == 0 || t + 1 == nsteps) save_ckpt(c, *A.ckpt[(t + ckpt_every) / ckpt_every]);
+        }
+        read_status(c);
+        if (c.h_status[kStCrash] & ~(uint32_t)VTP_CRASH_OUT_OF_GRID) {
+            c.crashed = true;
+            c.err = "solver crashed during the rollout";
+            return VTP_ECRASH;
+        }
+        A.valid = true;
+        return VTP_OK;
+    });
+}
+
+extern "C" vtp_status vtp_pbf_rollout_bwd(vtp_ctx *ctx, const float *grad_x, const float *grad_v, float *grad_v0,
+                                          float *grad_x0, float *grad_k) {
+    if (!ctx) return VTP_EINVAL;
+    Ctx &c = ctx->c;
+    vtp_status s = adj_check(c);
+    if (s != VTP_OK) return s;
+    AdjointState &A = adj(c);
+    if (!A.valid) { c.err = "vtp_pbf_rollout_bwd needs a preceding vtp_pbf_rollout_fwd"; return VTP_ESTATE; }
+    return adj_guarded(c, [&]() -> vtp_status {
+        const uint32_t n = c.n;
+        if (A.tape.size() < A.every) {
+            A.tape.clear();
+            for (uint32_t t = 0; t < A.every; ++t) {
+                auto tb = std::make_unique<TapeBufs>();
+                alloc_tape(c, *tb);
+                A.tape.push_back(std::move(tb));
+            }
+            A.xb.alloc(n);
+            A.vb.alloc(n);
+            A.xb2.alloc(n);
+            A.vb2.alloc(n);
+            A.U.alloc(n);
+            A.Wv.alloc(n);
+            A.rs.alloc(n);
+            A.kbar.alloc(1);
+            A.mat.alloc(n);
+        }
+        // terminal adjoints: attach order -> final slot order (non-fluid zero)
+        std::vector<uint32_t> orig(n);
+        std::vector<int32_t> mat(n);
+        VTP_HIP(hipMemcpyAsync(orig.data(), A.ckpt[A.nseg]->orig.p, n * 4, hipMemcpyDeviceToHost, c.stream));
+        VTP_HIP(hipStreamSynchronize(c.stream));
+        std::vector<float4> hx(n), hv(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            uint32_t a = orig[i];
+            bool fluid = c.h_mat[a] == VTP_MAT_FLUID;
+            hx[i] = (grad_x && fluid) ? make_float4(grad_x[3 * a], grad_x[3 * a + 1], grad_x[3 * a + 2], 0)
+                                      : make_float4(0, 0, 0, 0);
+            hv[i] = (grad_v && fluid) ? make_float4(grad_v[3 * a], grad_v[3 * a + 1], grad_v[3 * a + 2], 0)
+                                      : make_float4(0, 0, 0, 0);
+        }
+        VTP_HIP(hipMemcpyAsync(A.xb.p, hx.data(), n * sizeof(float4), hipMemcpyHostToDevice, c.stream));
+        VTP_HIP(hipMemcpyAsync(A.vb.p, hv.data(), n * sizeof(float4), hipMemcpyHostToDevice, c.stream));
+        VTP_HIP(hipMemsetAsync(A.kbar.p, 0, sizeof(double), c.stream));
+        for (int seg = (int)A.nseg - 1; seg >= 0; --seg) {
+            const uint32_t s0 = (uint32_t)seg * A.every;
+            const uint32_t len = std::min(A.every, A.nsteps - s0);
+            load_ckpt(c, *A.ckpt[seg]);
+            for (uint32_t t = 0; t < len; ++t) pbf_step_taped(c, A.tape[t]->ptrs);
+            for (int t = (int)len - 1; t >= 0; --t) reverse_tick(c, A, *A.tape[t]);
+        }
+        VTP_HIP(hipGetLastError());
+        // adjoints are now in the slot order of the rollout's first state
+        VTP_HIP(hipMemcpyAsync(orig.data(), A.ckpt[0]->orig.p, n * 4, hipMemcpyDeviceToHost, c.stream));
+        VTP_HIP(hipMemcpyAsync(hx.data(), A.xb.p, n * sizeof(float4), hipMemcpyDeviceToHost, c.stream));
+        VTP_HIP(hipMemcpyAsync(hv.data(), A.vb.p, n * sizeof(float4), hipMemcpyDeviceToHost, c.stream));
+        double kb = 0;
+        VTP_HIP(hipMemcpyAsync(&kb, A.kbar.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        // leave the context at the rollout's final state
+        load_ckpt(c, *A.ckpt[A.nseg]);
+        VTP_HIP(hipStreamSynchronize(c.stream));
+        for (uint32_t i = 0; i < n; ++i) {
+            uint32_t a = orig[i];
+            if (grad_v0) { grad_v0[3 * a] = hv[i].x; grad_v0[3 * a + 1] = hv[i].y; grad_v0[3 * a + 2] = hv[i].z; }
+            if (grad_x0) { grad_x0[3 * a] = hx[i].x; grad_x0[3 * a + 1] = hx[i].y; grad_x0[3 * a + 2] = hx[i].z; }
+        }
+        if (grad_k) *grad_k = (float)kb;
+        c.lists_stale = true;
+        return VTP_OK;
+    });
+}
+
+extern "C" vtp_status vtp_pbf_density_loss(vtp_ctx *ctx, double *loss, float *grad_x) {
+    if (!ctx) return VTP_EINVAL;
+    Ctx &c = ctx->c;
+    if (c.p.solver != VTP_PBF) { c.err = "density loss exists for PBF only"; return VTP_EINVAL; }
+    if (!c.initialized) { c.err = "not initialized"; return VTP_ESTATE; }
+    if (c.lists_stale) { c.err = "neighbour lists are stale after a rollout backward; tick first"; return VTP_ESTATE; }
+    return adj_guarded(c, [&]() -> vtp_status {
+        const uint32_t n = c.n, nb = div_up(n, 256);
+        const PbfK q = pbf_constants(c);
+        uint32_t nf = 0;
+        for (int32_t m : c.h_mat) nf += m == VTP_MAT_FLUID;
+        DevBuf<float> rb;
+        DevBuf<float4> g;
+        DevBuf<double> L;
+        rb.alloc(n);
+        g.alloc(n);
+        L.alloc(1);
+        VTP_HIP(hipMemsetAsync(L.p, 0, sizeof(double), c.stream));
+        Ugns &u = c.ugns;
+        hipLaunchKernelGGL(k_dl_rho, dim3(nb), dim3(256), 0, c.stream, c.X[0].p, c.mat.p, u.nbr.p, u.cnt.p, n,
+                           u.stride, q, nf ? 1.0f / nf : 0.0f, rb.p, L.p);
+        hipLaunchKernelGGL(k_dl_grad, dim3(nb), dim3(256), 0, c.stream, c.X[0].p, rb.p, c.mat.p, u.nbr.p, u.cnt.p, n,
+                           u.stride, q, g.p);
+        VTP_HIP(hipGetLastError());
+        double lsum = 0;
+        VTP_HIP(hipMemcpyAsync(&lsum, L.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        std::vector<float4> hg(n);
+        std::vector<uint32_t> orig(n);
+        if (grad_x) {
+            VTP_HIP(hipMemcpyAsync(hg.data(), g.p, n * sizeof(float4), hipMemcpyDeviceToHost, c.stream));
+            VTP_HIP(hipMemcpyAsync(orig.data(), c.orig.p, n * 4, hipMemcpyDeviceToHost, c.stream));
+        }
+        VTP_HIP(hipStreamSynchronize(c.stream));
+        if (loss) *loss = nf ? lsum / nf : 0.0;
+        if (grad_x)
+            for (uint32_t i = 0; i < n; ++i) {
+                uint32_t a = orig[i];
+                grad_x[3 * a] = hg[i].x;
+                grad_x[3 * a + 1] = hg[i].y;
+                grad_x[3 * a + 2] = hg[i].z;
+            }
+        return VTP_OK;
+    });
+}
+
+extern "C" vtp_status vtp_pbf_set_xsph(vtp_ctx *ctx, float xsph_k) {
+    if (!ctx) return VTP_EINVAL;
+    Ctx &c = ctx->c;
+    if (c.p.solver != VTP_PBF) { c.err = "XSPH_k is a PBF parameter"; return VTP_EINVAL; }
+    c.p.xsph_k = xsph_k;
+    pbf_capture_reset(c);   // the captured graph holds the old constant
+    return VTP_OK;
+}

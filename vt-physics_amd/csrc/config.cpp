@@ -9,7 +9,8 @@
 //
 // Extension keys (no reference counterpart) live in an optional
 // "<Solver>.Extensions" object: cflEnable, cflNumber, dtMax,
-// maxSolverIterations, polymerPhase, phaseRestViscosity (IMM-CT only).
+// maxSolverIterations, polymerPhase, phaseRestViscosity (IMM-CT only),
+// massWeighted (neighbour mass in the PBF gradient / IMM-CT stress term).
 #include <cstring>
 #include <string>
 
@@ -188,6 +189,7 @@ extern "C" vtp_status vtp_params_from_json(int32_t solver, const char *json_text
             if (e.contains("maxSolverIterations"))
                 p.max_solver_iterations = uint32_t(e["maxSolverIterations"].number());
             if (e.contains("polymerPhase")) p.polymer_phase = int32_t(e["polymerPhase"].number());
+            if (e.contains("massWeighted")) p.mass_weighted = e["massWeighted"].boolean() ? 1 : 0;
             if (e.contains("phaseRestViscosity") && solver == VTP_IMMCT) {
                 auto vis = e["phaseRestViscosity"].floats();
                 if (vis.size() != p.phase_num)

@@ -180,10 +180,6 @@ static void derive(Ctx &c) {
 
 using namespace vtp;
 
-struct vtp_ctx {
-    Ctx c;
-};
-
 static thread_local std::string g_create_error;
 
 #define CTX_GUARD(ctx)                                                         \
@@ -471,7 +467,7 @@ extern "C" vtp_status vtp_read(vtp_ctx *ctx, int32_t field, void *host_dst, size
             case VTP_FIELD_DENSITY: f1(c.dens.p); break;
             case VTP_FIELD_LAMBDA: f1(c.lam.p); break;
             case VTP_FIELD_ERROR: f1(c.err_c.p); break;
-            case VTP_FIELD_REST_DENSITY: f1(c.rest_density.p); break;
+            case VTP_FIELD_REST_DENSITY: f1(c.P ? c.rest2[0].p : c.rest_density.p); break;
             case VTP_FIELD_COMPRESSION_RATIO: f1(c.cr.p); break;
             case VTP_FIELD_DF_ALPHA: f1(c.alpha.p); break;
             case VTP_FIELD_DELTA_CR: f1(c.delta.p); break;
@@ -483,11 +479,12 @@ extern "C" vtp_status vtp_read(vtp_ctx *ctx, int32_t field, void *host_dst, size
             case VTP_FIELD_VOL_FRAC: phased(c.frac[0].p, 1, 1); break;
             case VTP_FIELD_VEL_PHASE: phased(c.uphase[0].p, 4, 3); break;
             case VTP_FIELD_VEL_DRIFT: phased(c.drift.p, 4, 3); break;
-            case VTP_FIELD_COLOR: f4_xyz(c.color.p); break;
-            case VTP_FIELD_CD: f1(c.cd.p); break;
-            case VTP_FIELD_Q: mat9(c.Q[0].p); break;
-            case VTP_FIELD_VEL_GRAD: mat9(c.Lgrad.p); break;
-            case VTP_FIELD_VISE_STRESS: mat9(c.stress.p); break;
+            case VTP_FIELD_COLOR: f4_xyz(c.P ? c.color4[0].p : c.color.p); break;
+            case VTP_FIELD_CD: f1(c.P ? c.cd.p : nullptr); break;
+            case VTP_FIELD_Q: case VTP_FIELD_VEL_GRAD: case VTP_FIELD_VISE_STRESS:
+                if (c.p.solver != VTP_IMMCT) throw std::runtime_error("field only exists for IMM-CT");
+                mat9(field == VTP_FIELD_Q ? c.Q[0].p : field == VTP_FIELD_VEL_GRAD ? c.Lgrad.p : c.stress.p);
+                break;
             default: throw std::runtime_error("field not available");
         }
         return VTP_OK;

@@ -151,20 +151,20 @@ __global__ __launch_bounds__(256) void k_df_precompute(const float4 *__restrict_
     const uint32_t nn = w & ~kDeadBit;
     float c = (w & kDeadBit) ? 0.0f : vi.w * q.k.w0;
     float a1x = 0, a1y = 0, a1z = 0, a2 = 1e-6f;
-    for (uint32_t t = 0; t < nn; ++t) {
-        uint32_t j = nbr[(size_t)t * stride + i];
-        float4 xj = x[j];
-        float Vj = v[j].w;
-        float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        float W, f;
-        cubic_w_grad(dx * dx + dy * dy + dz * dz, q.k, W, f);
-        c = fmaf(Vj, W, c);
-        float gx = dx * f, gy = dy * f, gz = dz * f;
-        a1x = fmaf(Vj, gx, a1x);
-        a1y = fmaf(Vj, gy, a1y);
-        a1z = fmaf(Vj, gz, a1z);
-        if (mat[j] == VTP_MAT_FLUID) a2 += (gx * gx + gy * gy + gz * gz) * Vj * Vj / xj.w;
-    }
+    struct D { float4 x; float V; int32_t m; };
+    for_each_neighbor(
+        nbr, stride, i, nn, [&](uint32_t j) { return D{x[j], v[j].w, mat[j]}; },
+        [&](const D &d) {
+            float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
+            float W, f;
+            cubic_w_grad(dx * dx + dy * dy + dz * dz, q.k, W, f);
+            c = fmaf(d.V, W, c);
+            float gx = dx * f, gy = dy * f, gz = dz * f;
+            a1x = fmaf(d.V, gx, a1x);
+            a1y = fmaf(d.V, gy, a1y);
+            a1z = fmaf(d.V, gz, a1z);
+            if (d.m == VTP_MAT_FLUID) a2 += (gx * gx + gy * gy + gz * gz) * d.V * d.V / d.x.w;
+        });
     float a = (a1x * a1x + a1y * a1y + a1z * a1z) / xi.w + a2;
     cr[i] = c;
     alpha[i] = fmaxf(a, 1e-6f);
@@ -206,16 +206,16 @@ __global__ __launch_bounds__(256) void k_df_delta_kappa(const float4 *__restrict
             const float4 ai = vadv[i];
             const uint32_t nn = cnt[i] & ~kDeadBit;
             float d = 0.0f;
-            for (uint32_t t = 0; t < nn; ++t) {
-                uint32_t j = nbr[(size_t)t * stride + i];
-                float4 xj = x[j];
-                float4 aj = vadv[j];
-                float Vj = v[j].w;
-                float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                float f = cubic_grad_factor(dx * dx + dy * dy + dz * dz, k);
-                float dot = (dx * (ai.x - aj.x) + dy * (ai.y - aj.y) + dz * (ai.z - aj.z)) * f;
-                d = fmaf(dot, Vj, d);
-            }
+            struct D { float4 x, a; };
+            for_each_neighbor(
+                nbr, stride, i, nn,
+                [&](uint32_t j) { return D{x[j], make_float4(vadv[j].x, vadv[j].y, vadv[j].z, v[j].w)}; },
+                [&](const D &e) {
+                    float dx = xi.x - e.x.x, dy = xi.y - e.x.y, dz = xi.z - e.x.z;
+                    float f = cubic_grad_factor(dx * dx + dy * dy + dz * dz, k);
+                    float dot = (dx * (ai.x - e.a.x) + dy * (ai.y - e.a.y) + dz * (ai.z - e.a.z)) * f;
+                    d = fmaf(dot, e.a.w, d);
+                });
             float dl = fmaxf(cr[i] - 1.0f + d * dt, 0.0f);
             float inv_dt = 1.0f / dt;
             delta[i] = dl;
@@ -258,16 +258,16 @@ __global__ __launch_bounds__(256) void k_df_update_vadv(const float4 *__restrict
     const float ki = kappa[i];
     const uint32_t nn = cnt[i] & ~kDeadBit;
     float ax = 0, ay = 0, az = 0;
-    for (uint32_t t = 0; t < nn; ++t) {
-        uint32_t j = nbr[(size_t)t * stride + i];
-        float4 xj = x[j];
-        float Vj = v[j].w;
-        float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        float s = cubic_grad_factor(dx * dx + dy * dy + dz * dz, k) * Vj * (ki + kappa[j]);
-        ax = fmaf(dx, s, ax);
-        ay = fmaf(dy, s, ay);
-        az = fmaf(dz, s, az);
-    }
+    struct D { float4 x; float k; };
+    for_each_neighbor(
+        nbr, stride, i, nn, [&](uint32_t j) { return D{make_float4(x[j].x, x[j].y, x[j].z, v[j].w), kappa[j]}; },
+        [&](const D &e) {
+            float dx = xi.x - e.x.x, dy = xi.y - e.x.y, dz = xi.z - e.x.z;
+            float s = cubic_grad_factor(dx * dx + dy * dy + dz * dz, k) * e.x.w * (ki + e.k);
+            ax = fmaf(dx, s, ax);
+            ay = fmaf(dy, s, ay);
+            az = fmaf(dz, s, az);
+        });
     float c = dt * Vi / xi.w;
     float4 a = vadv[i];
     a.x -= c * ax;
@@ -363,26 +363,29 @@ __global__ __launch_bounds__(256) void k_df_vis_normal(const float4 *__restrict_
     const uint32_t nn = cnt[i] & ~kDeadBit;
     float ax = q.gx, ay = q.gy, az = q.gz;
     float nx = 0, ny = 0, nz = 0;
-    for (uint32_t t = 0; t < nn; ++t) {
-        uint32_t j = nbr[(size_t)t * stride + i];
-        if (mat[j] != mi) continue;
-        float4 xj = x[j];
-        float4 vj = v[j];
-        float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        float r2 = dx * dx + dy * dy + dz * dz;
-        float f = cubic_grad_factor(r2, q.k);
-        float mr = xj.w / rest[j];
-        float vdx = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
-        float pi = 0.5f * (visi + vis[j]) * mr * vdx / (r2 + q.h2_001);
-        float s = 10.0f * pi * f;
-        ax = fmaf(dx, s, ax);
-        ay = fmaf(dy, s, ay);
-        az = fmaf(dz, s, az);
-        float sn = q.k.h * mr * f;
-        nx = fmaf(dx, sn, nx);
-        ny = fmaf(dy, sn, ny);
-        nz = fmaf(dz, sn, nz);
-    }
+    struct D { float4 x, v; float mr, vis; int32_t m; };
+    for_each_neighbor(
+        nbr, stride, i, nn,
+        [&](uint32_t j) {
+            float4 xj = x[j];
+            return D{xj, v[j], xj.w / rest[j], vis[j], mat[j]};
+        },
+        [&](const D &d) {
+            if (d.m != mi) return;
+            float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            float f = cubic_grad_factor(r2, q.k);
+            float vdx = (vi.x - d.v.x) * dx + (vi.y - d.v.y) * dy + (vi.z - d.v.z) * dz;
+            float pi = 0.5f * (visi + d.vis) * d.mr * vdx / (r2 + q.h2_001);
+            float s = 10.0f * pi * f;
+            ax = fmaf(dx, s, ax);
+            ay = fmaf(dy, s, ay);
+            az = fmaf(dz, s, az);
+            float sn = q.k.h * d.mr * f;
+            nx = fmaf(dx, sn, nx);
+            ny = fmaf(dy, sn, ny);
+            nz = fmaf(dz, sn, nz);
+        });
     acc[i] = make_float4(ax, ay, az, 0.0f);
     normal[i] = make_float4(nx, ny, nz, 0.0f);
 }
@@ -410,20 +413,20 @@ __global__ __launch_bounds__(256) void k_df_tension_apply(const float4 *__restri
         const float ri = rest[i];
         const uint32_t nn = cnt[i] & ~kDeadBit;
         float sx = 0, sy = 0, sz = 0;
-        for (uint32_t t = 0; t < nn; ++t) {
-            uint32_t j = nbr[(size_t)t * stride + i];
-            if (mat[j] != mi) continue;
-            float4 xj = x[j];
-            float4 nj = normal[j];
-            float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-            float r = sqrtf(dx * dx + dy * dy + dz * dz);
-            float kk = 2.0f * ri / (ri + rest[j]);
-            float c1 = -q.gamma * xi.w * xj.w * surface_tension_c(r, q.k.h, q.c_sigma, q.h6_64) / r;
-            float c2 = -q.gamma * xi.w;
-            sx += kk * (c1 * dx + c2 * (ni.x - nj.x));
-            sy += kk * (c1 * dy + c2 * (ni.y - nj.y));
-            sz += kk * (c1 * dz + c2 * (ni.z - nj.z));
-        }
+        struct D { float4 x, n; float r; int32_t m; };
+        for_each_neighbor(
+            nbr, stride, i, nn, [&](uint32_t j) { return D{x[j], normal[j], rest[j], mat[j]}; },
+            [&](const D &d) {
+                if (d.m != mi) return;
+                float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
+                float r = sqrtf(dx * dx + dy * dy + dz * dz);
+                float kk = 2.0f * ri / (ri + d.r);
+                float c1 = -q.gamma * xi.w * d.x.w * surface_tension_c(r, q.k.h, q.c_sigma, q.h6_64) / r;
+                float c2 = -q.gamma * xi.w;
+                sx += kk * (c1 * dx + c2 * (ni.x - d.n.x));
+                sy += kk * (c1 * dy + c2 * (ni.y - d.n.y));
+                sz += kk * (c1 * dz + c2 * (ni.z - d.n.z));
+            });
         a.x += sx;
         a.y += sy;
         a.z += sz;

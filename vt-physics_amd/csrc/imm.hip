@@ -1,6 +1,887 @@
-// imm.hip - placeholder until the IMM / IMM-CT kernels land.
+// imm.hip - IMM mixture and IMM-CT viscoelastic steps
+// (IMMCudaImpl.cu, IMMSolver.cpp:400-479, IMMCTCudaImpl.cu, IMMCTSolver.cpp:408-492).
+//
+// Per-phase arrays are phase-major, a[k*stride + slot], so every phase of a
+// wavefront's particles is one coalesced access.  The carried state (phase
+// fractions, phase velocities, rest density, colour, conformation tensor) is
+// gathered into the new slot order after each sort.  Passes:
+//   precompute (cr, alpha, v* = v, IMM-CT Cd) -> div solve -> pressure to phases
+//   -> viscosity + normal -> tension + phase velocities
+//   -> [IMM-CT] velocity gradient + rheology + conformation update -> polymer stress acc
+//   -> density solve -> pressure to phases + update_pos
+//   -> phase fluxes (drift + diffusion) with the negative-fraction retry loop
+//   -> fraction update / regularise / rest density / mixture velocity / colour
+//   -> boundary rest density (max over fluid neighbours).
+// Generalisation: P phases (<= 8) and a configurable polymer phase (the
+// reference IMM-CT is two-phase with index 1 hard-coded).
 #include "vtp_internal.h"
+
 namespace vtp {
-void imm_initialize(Ctx &) { throw std::runtime_error("IMM not built yet"); }
-void imm_step(Ctx &) { throw std::runtime_error("IMM not built yet"); }
+
+struct ImmK {
+    Cubic k;
+    float gx, gy, gz;
+    float gamma, c_sigma, h6_64;
+    float cf, cd0;
+    float rho[VTP_MAX_PHASES], mu[VTP_MAX_PHASES];
+    float col[VTP_MAX_PHASES][3];
+    float vis_base, vis_max, inv_relax, beta0, theta;
+    int P, pp, ct;
+    float V0;
+    int mass_weighted;   // extension: m_j in the polymer-stress term
+};
+
+static ImmK imm_constants(const Ctx &c) {
+    ImmK q{};
+    q.k = c.kern;
+    q.gx = c.p.gravity[0];
+    q.gy = c.p.gravity[1];
+    q.gz = c.p.gravity[2];
+    q.gamma = c.p.surface_tension_coefficient;
+    const float pi = 3.14159265f;
+    q.c_sigma = 32.0f / pi / std::pow(c.h, 9.0f);
+    q.h6_64 = std::pow(c.h, 6.0f) / 64.0f;
+    q.cf = c.p.cf;
+    q.cd0 = c.p.cd;
+    q.P = (int)c.p.phase_num;
+    for (int k = 0; k < q.P; ++k) {
+        q.rho[k] = c.p.phase_rest_density[k];
+        q.mu[k] = c.p.phase_rest_viscosity[k];
+        for (int a = 0; a < 3; ++a) q.col[k][a] = c.p.phase_rest_color[k][a];
+    }
+    // the reference scales these by 1000 on the device (IMMCTCudaImpl.cu:58-61)
+    q.vis_base = 1000.0f * c.p.solution_vis_base;
+    q.vis_max = 1000.0f * c.p.solution_vis_max;
+    q.inv_relax = c.p.relaxation_time > 0.f ? 1.0f / c.p.relaxation_time : 0.f;
+    q.beta0 = c.p.thinning_exp0;
+    q.theta = c.p.rheology_threshold;
+    q.pp = c.p.polymer_phase;
+    q.ct = c.p.solver == VTP_IMMCT ? 1 : 0;
+    q.V0 = c.V0;
+    q.mass_weighted = c.p.mass_weighted;
+    return q;
+}
+
+struct Ph {   // phase-major arrays
+    float *frac;
+    float4 *u;
+    float4 *a;
+    float4 *drift;
+    float *fin, *fout;
+    uint32_t stride;
+};
+
+__device__ __forceinline__ float4 f4(float x, float y, float z) { return make_float4(x, y, z, 0.0f); }
+
+// mixture velocity from phase velocities (update_vel_from_phase_vel_cuda,
+// IMMCudaImpl.cu:396-415): v = sum_k frac_k u_k, v* = v, drift_k = u_k - v.
+template <int MP>
+__device__ __forceinline__ float4 mix_velocity(const Ph &ph, uint32_t i, int P, float4 vold, float4 *vadv) {
+    float vx = 0, vy = 0, vz = 0;
+#pragma unroll
+    for (int k = 0; k < MP; ++k) {
+        if (k >= P) break;
+        float a = ph.frac[(size_t)k * ph.stride + i];
+        float4 u = ph.u[(size_t)k * ph.stride + i];
+        vx = fmaf(u.x, a, vx);
+        vy = fmaf(u.y, a, vy);
+        vz = fmaf(u.z, a, vz);
+    }
+#pragma unroll
+    for (int k = 0; k < MP; ++k) {
+        if (k >= P) break;
+        float4 u = ph.u[(size_t)k * ph.stride + i];
+        ph.drift[(size_t)k * ph.stride + i] = f4(u.x - vx, u.y - vy, u.z - vz);
+    }
+    float4 v = make_float4(vx, vy, vz, vold.w);
+    vadv[i] = v;
+    return v;
+}
+
+// ------------------------------------------------------------------ init
+// init_data_cuda + prepare_imm (IMMCudaImpl.cu:51-118, 684-701): V = V0,
+// u_k = v, rest = sum_k frac_k rho_k, boundary rest raised to the fluid
+// neighbours' maximum (in a second pass: race-free), mass = rest * V0 for
+// every particle (prepare_imm runs before the boundary volume is known),
+// then the boundary volume 1/sum W.  IMM-CT: Q = I, Cd = Cd0, vis = base.
+template <int MP>
+__global__ void k_imm_init_a(float4 *__restrict__ x, float4 *__restrict__ v, float4 *__restrict__ vadv,
+                             float *__restrict__ rest, float *__restrict__ cd, float *__restrict__ vis,
+                             float *__restrict__ Q, float4 *__restrict__ color, const int32_t *__restrict__ mat,
+                             uint32_t n, ImmK q, Ph ph) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 vi = v[i];
+    vi.w = q.V0;
+    v[i] = vi;
+    vadv[i] = vi;
+    float r = 0.0f, cx = 0, cy = 0, cz = 0;
+#pragma unroll
+    for (int k = 0; k < MP; ++k) {
+        if (k >= q.P) break;
+        float a = ph.frac[(size_t)k * ph.stride + i];
+        r += a * q.rho[k];
+        ph.u[(size_t)k * ph.stride + i] = f4(vi.x, vi.y, vi.z);
+        ph.drift[(size_t)k * ph.stride + i] = f4(0, 0, 0);
+        ph.fin[(size_t)k * ph.stride + i] = 0.0f;
+        ph.fout[(size_t)k * ph.stride + i] = 0.0f;
+        cx += a * q.col[k][0];
+        cy += a * q.col[k][1];
+        cz += a * q.col[k][2];
+    }
+    rest[i] = r;
+    color[i] = f4(cx, cy, cz);
+    cd[i] = q.cd0;
+    if (q.ct) {
+        vis[i] = q.vis_base;
+#pragma unroll
+        for (int a = 0; a < 9; ++a) Q[(size_t)a * ph.stride + i] = (a == 0 || a == 4 || a == 8) ? 1.0f : 0.0f;
+    }
+    (void)x;
+    (void)mat;
+}
+
+// Boundary rest density: max(own, fluid neighbours) and mass = rest * V.
+// With `volume_pass` the boundary volume 1/sum_{same material} W is computed
+// afterwards (prepare_imm order: mass uses V before the volume update).
+__global__ void k_imm_boundary_rest(float4 *__restrict__ x, float4 *__restrict__ v, const float *__restrict__ rest_in,
+                                    float *__restrict__ rest, const int32_t *__restrict__ mat,
+                                    const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ cnt, uint32_t n,
+                                    uint32_t stride, Cubic k, int volume_pass) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t mi = mat[i];
+    const uint32_t nn = cnt[i] & ~kDeadBit;
+    if (mi == VTP_MAT_FLUID) {
+        if (volume_pass) x[i].w = rest_in[i] * v[i].w;
+        return;
+    }
+    float r = rest_in[i];
+    for (uint32_t t = 0; t < nn; ++t) {
+        uint32_t j = nbr[(size_t)t * stride + i];
+        if (mat[j] == VTP_MAT_FLUID) r = fmaxf(r, rest_in[j]);
+    }
+    rest[i] = r;
+    float4 vi = v[i];
+    x[i].w = r * vi.w;
+    if (volume_pass && mi == VTP_MAT_BOUNDARY) {
+        const float4 xi = x[i];
+        float d = (cnt[i] & kDeadBit) ? 0.0f : k.w0;
+        for (uint32_t t = 0; t < nn; ++t) {
+            uint32_t j = nbr[(size_t)t * stride + i];
+            if (mat[j] != mi) continue;
+            float4 xj = x[j];
+            float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            d += cubic_w(dx * dx + dy * dy + dz * dz, k);
+        }
+        vi.w = 1.0f / d;
+        v[i] = vi;
+    }
+}
+
+// ------------------------------------------------------------------ precompute
+// compute_compression_ratio + compute_df_beta (IMMCudaImpl.cu:120-175), v* = v,
+// and for IMM-CT update_Cd_cuda (IMMCTCudaImpl.cu:782-804):
+// Cd = Cd0 + (1 - Cd0) sum_{j same material, j != i} V_j frac_j,p W_ij.
+__global__ __launch_bounds__(256) void k_imm_precompute(const float4 *__restrict__ x, const float4 *__restrict__ v,
+                                                        float4 *__restrict__ vadv, const int32_t *__restrict__ mat,
+                                                        const uint32_t *__restrict__ nbr,
+                                                        const uint32_t *__restrict__ cnt, uint32_t n,
+                                                        uint32_t stride, ImmK q, const float *__restrict__ fpoly,
+                                                        float *__restrict__ cr, float *__restrict__ alpha,
+                                                        float *__restrict__ cd) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 vi = v[i];
+    vadv[i] = vi;
+    const int32_t mi = mat[i];
+    if (mi != VTP_MAT_FLUID) {
+        cr[i] = 0.0f;
+        alpha[i] = 0.0f;
+        if (q.ct) cd[i] = q.cd0;
+        return;
+    }
+    const float4 xi = x[i];
+    const uint32_t w = cnt[i];
+    const uint32_t nn = w & ~kDeadBit;
+    float c = (w & kDeadBit) ? 0.0f : vi.w * q.k.w0;
+    float a1x = 0, a1y = 0, a1z = 0, a2 = 1e-6f, b = 0.0f;
+    struct D { float4 x; float V, fp; int32_t m; };
+    for_each_neighbor(
+        nbr, stride, i, nn, [&](uint32_t j) { return D{x[j], v[j].w, q.ct ? fpoly[j] : 0.0f, mat[j]}; },
+        [&](const D &d) {
+            float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
+            float W, f;
+            cubic_w_grad(dx * dx + dy * dy + dz * dz, q.k, W, f);
+            c = fmaf(d.V, W, c);
+            float gx = dx * f, gy = dy * f, gz = dz * f;
+            a1x = fmaf(d.V, gx, a1x);
+            a1y = fmaf(d.V, gy, a1y);
+            a1z = fmaf(d.V, gz, a1z);
+            if (d.m == VTP_MAT_FLUID) a2 += (gx * gx + gy * gy + gz * gz) * d.V * d.V / d.x.w;
+            if (d.m == mi) b = fmaf(d.V * d.fp, W, b);
+        });
+    cr[i] = c;
+    alpha[i] = fmaxf((a1x * a1x + a1y * a1y + a1z * a1z) / xi.w + a2, 1e-6f);
+    if (q.ct) cd[i] = q.cd0 + (1.0f - q.cd0) * b;
+}
+
+// ------------------------------------------------------------------ pressure -> phases
+// apply_pressure_acc (IMMCudaImpl.cu:770-788): acc = (v* - v)/dt; a_k = acc (Cd +
+// (1-Cd) rest/rho_k); u_k += a_k dt; mixture velocity.  With `xo` set it also
+// runs update_pos (:471-482) in place.
+template <int MP>
+__global__ void k_imm_pressure_to_phases(float4 *__restrict__ x, float4 *__restrict__ v,
+                                         float4 *__restrict__ vadv, float4 *__restrict__ acc,
+                                         const int32_t *__restrict__ mat, const float *__restrict__ rest,
+                                         const float *__restrict__ cd, uint32_t n, ImmK q, Ph ph,
+                                         const SolverScalars *sc, int update_pos) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float dt = sc->dt, inv_dt = 1.0f / dt;
+    float4 vi = v[i];
+    float4 a = vadv[i];
+    float4 ac = f4((a.x - vi.x) * inv_dt, (a.y - vi.y) * inv_dt, (a.z - vi.z) * inv_dt);
+    acc[i] = ac;
+    if (mat[i] != VTP_MAT_FLUID) return;
+    const float ci = cd[i], ri = rest[i];
+#pragma unroll
+    for (int k = 0; k < MP; ++k) {
+        if (k >= q.P) break;
+        float s = ci + (1.0f - ci) * (ri / q.rho[k]);
+        size_t o = (size_t)k * ph.stride + i;
+        float4 u = ph.u[o];
+        u.x += ac.x * s * dt;
+        u.y += ac.y * s * dt;
+        u.z += ac.z * s * dt;
+        ph.u[o] = u;
+    }
+    vi = mix_velocity<MP>(ph, i, q.P, vi, vadv);
+    v[i] = vi;
+    if (update_pos) {
+        float4 xi = x[i];
+        xi.x += vi.x * dt;
+        xi.y += vi.y * dt;
+        xi.z += vi.z * dt;
+        x[i] = xi;
+    }
+}
+
+// ------------------------------------------------------------------ viscosity + normal
+// add_phase_acc_gravity + add_phase_acc_vis (IMMCudaImpl.cu:275-321) and
+// compute_surface_normal (:323-346):
+//   a_k = g + sum_{j same mat, j != i} 10 V_j ((mu_k(1-Cd)(u_k - v_j) + mu_k Cd v_ij) . x_ij) gradW / |x_ij|^2
+//   n_i = h sum_{j same mat} m_j/rest_j gradW.
+template <int MP>
+__global__ __launch_bounds__(256) void k_imm_vis_normal(const float4 *__restrict__ x, const float4 *__restrict__ v,
+                                                        const int32_t *__restrict__ mat,
+                                                        const float *__restrict__ rest, const float *__restrict__ cd,
+                                                        const uint32_t *__restrict__ nbr,
+                                                        const uint32_t *__restrict__ cnt, uint32_t n,
+                                                        uint32_t stride, ImmK q, Ph ph,
+                                                        float4 *__restrict__ normal) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t mi = mat[i];
+    if (mi != VTP_MAT_FLUID) return;
+    const float4 xi = x[i];
+    const float4 vi = v[i];
+    const float ci = cd[i];
+    const uint32_t nn = cnt[i] & ~kDeadBit;
+    float4 uk[MP];
+    float ax[MP], ay[MP], az[MP];
+#pragma unroll
+    for (int k = 0; k < MP; ++k) {
+        uk[k] = k < q.P ? ph.u[(size_t)k * ph.stride + i] : f4(0, 0, 0);
+        ax[k] = q.gx;
+        ay[k] = q.gy;
+        az[k] = q.gz;
+    }
+    float nx = 0, ny = 0, nz = 0;
+    struct D { float4 x, v; float r; int32_t m; };
+    for_each_neighbor(
+        nbr, stride, i, nn, [&](uint32_t j) { return D{x[j], v[j], rest[j], mat[j]}; },
+        [&](const D &d) {
+            if (d.m != mi) return;
+            float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            float f = cubic_grad_factor(r2, q.k);
+            float vijx = (vi.x - d.v.x) * dx + (vi.y - d.v.y) * dy + (vi.z - d.v.z) * dz;
+            float base = 10.0f * d.v.w * f / r2;
+#pragma unroll
+            for (int k = 0; k < MP; ++k) {
+                if (k >= q.P) break;
+                float ukx = (uk[k].x - d.v.x) * dx + (uk[k].y - d.v.y) * dy + (uk[k].z - d.v.z) * dz;
+                float s = (q.mu[k] * (1.0f - ci) * ukx + q.mu[k] * ci * vijx) * base;
+                ax[k] = fmaf(dx, s, ax[k]);
+                ay[k] = fmaf(dy, s, ay[k]);
+                az[k] = fmaf(dz, s, az[k]);
+            }
+            float sn = q.k.h * d.x.w / d.r * f;
+            nx = fmaf(dx, sn, nx);
+            ny = fmaf(dy, sn, ny);
+            nz = fmaf(dz, sn, nz);
+        });
+#pragma unroll
+    for (int k = 0; k < MP; ++k)
+        if (k < q.P) ph.a[(size_t)k * ph.stride + i] = f4(ax[k], ay[k], az[k]);
+    normal[i] = f4(nx, ny, nz);
+}
+
+// add_phase_acc_surface_tension (IMMCudaImpl.cu:348-381) + phase_acc_2_phase_vel
+// + update_vel_from_phase_vel.
+template <int MP>
+__global__ __launch_bounds__(256) void k_imm_tension_phase_vel(const float4 *__restrict__ x, float4 *__restrict__ v,
+                                                               float4 *__restrict__ vadv,
+                                                               const int32_t *__restrict__ mat,
+                                                               const float *__restrict__ rest,
+                                                               const float4 *__restrict__ normal,
+                                                               const uint32_t *__restrict__ nbr,
+                                                               const uint32_t *__restrict__ cnt, uint32_t n,
+                                                               uint32_t stride, ImmK q, Ph ph,
+                                                               const SolverScalars *sc) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t mi = mat[i];
+    if (mi != VTP_MAT_FLUID) return;
+    const float dt = sc->dt;
+    float sx = 0, sy = 0, sz = 0;
+    if (q.gamma != 0.0f) {
+        const float4 xi = x[i];
+        const float4 ni = normal[i];
+        const float ri = rest[i];
+        const uint32_t nn = cnt[i] & ~kDeadBit;
+        struct D { float4 x, n; float r; int32_t m; };
+        for_each_neighbor(
+            nbr, stride, i, nn, [&](uint32_t j) { return D{x[j], normal[j], rest[j], mat[j]}; },
+            [&](const D &d) {
+                if (d.m != mi) return;
+                float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
+                float r = sqrtf(dx * dx + dy * dy + dz * dz);
+                float kk = 2.0f * ri / (ri + d.r);
+                float c1 = -q.gamma * xi.w * d.x.w * surface_tension_c(r, q.k.h, q.c_sigma, q.h6_64) / r;
+                float c2 = -q.gamma * xi.w;
+                sx += kk * (c1 * dx + c2 * (ni.x - d.n.x));
+                sy += kk * (c1 * dy + c2 * (ni.y - d.n.y));
+                sz += kk * (c1 * dz + c2 * (ni.z - d.n.z));
+            });
+    }
+#pragma unroll
+    for (int k = 0; k < MP; ++k) {
+        if (k >= q.P) break;
+        size_t o = (size_t)k * ph.stride + i;
+        float4 a = ph.a[o];
+        float4 u = ph.u[o];
+        u.x += (a.x + sx) * dt;
+        u.y += (a.y + sy) * dt;
+        u.z += (a.z + sz) * dt;
+        ph.u[o] = u;
+    }
+    v[i] = mix_velocity<MP>(ph, i, q.P, v[i], vadv);
+}
+
+// ------------------------------------------------------------------ IMM-CT
+// compute_vel_grad (IMMCTCudaImpl.cu:687-714) + compute_rheological_factors
+// (:806-821) + update_ct (:716-744), one pass:
+//   L = sum_{j fluid} m_j/rest_j (v_j - v_i) (x) gradW_ij   (row from v, column from gradW),
+//   beta = beta0 max(0, theta - a_p)/(theta + 1e-6),  kappa = (1 - a_p)^-1.2,
+//   Q += dt (Q L + L^T Q - (Q - I)/lambda - beta (Q - I) Q),
+//   D = (L + L^T)/2, shear = sqrt(tr(D)^2 / 2),
+//   mu = min(base kappa, max);  mu = base + (mu - base)/(1 + (10 shear)^5) min(1, a_p/(theta + 1e-6)),
+//   stress = mu (Q - I).
+__global__ __launch_bounds__(256) void k_immct_conformation(const float4 *__restrict__ x,
+                                                            const float4 *__restrict__ v,
+                                                            const int32_t *__restrict__ mat,
+                                                            const float *__restrict__ rest,
+                                                            const uint32_t *__restrict__ nbr,
+                                                            const uint32_t *__restrict__ cnt, uint32_t n,
+                                                            uint32_t stride, ImmK q, const float *__restrict__ fpoly,
+                                                            float *__restrict__ Q, float *__restrict__ Lout,
+                                                            float *__restrict__ stress, float *__restrict__ vis,
+                                                            const SolverScalars *sc) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (mat[i] != VTP_MAT_FLUID) {
+        vis[i] = q.vis_base;
+        return;
+    }
+    const float dt = sc->dt;
+    const float4 xi = x[i];
+    const float4 vi = v[i];
+    const uint32_t nn = cnt[i] & ~kDeadBit;
+    float L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    struct D { float4 x, v; float r; int32_t m; };
+    for_each_neighbor(
+        nbr, stride, i, nn, [&](uint32_t j) { return D{x[j], v[j], rest[j], mat[j]}; },
+        [&](const D &d) {
+            if (d.m != VTP_MAT_FLUID) return;
+            float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
+            float f = cubic_grad_factor(dx * dx + dy * dy + dz * dz, q.k) * (d.x.w / d.r);
+            float gx = dx * f, gy = dy * f, gz = dz * f;
+            float ux = d.v.x - vi.x, uy = d.v.y - vi.y, uz = d.v.z - vi.z;
+            L[0] = fmaf(ux, gx, L[0]); L[1] = fmaf(ux, gy, L[1]); L[2] = fmaf(ux, gz, L[2]);
+            L[3] = fmaf(uy, gx, L[3]); L[4] = fmaf(uy, gy, L[4]); L[5] = fmaf(uy, gz, L[5]);
+            L[6] = fmaf(uz, gx, L[6]); L[7] = fmaf(uz, gy, L[7]); L[8] = fmaf(uz, gz, L[8]);
+        });
+    const size_t st = stride;
+    float Qm[9];
+#pragma unroll
+    for (int a = 0; a < 9; ++a) Qm[a] = Q[a * st + i];
+    const float ap = fpoly[i];
+    const float beta = q.beta0 * fmaxf(0.0f, q.theta - ap) / (q.theta + 1e-6f);
+    const float kap = powf(1.0f - ap, -1.2f);
+    float QI[9];
+#pragma unroll
+    for (int a = 0; a < 9; ++a) QI[a] = Qm[a] - ((a == 0 || a == 4 || a == 8) ? 1.0f : 0.0f);
+    float Qn[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            float ql = 0, ltq = 0, qiq = 0;
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                ql += Qm[3 * r + m] * L[3 * m + cc];      // (Q L)_rc
+                ltq += L[3 * m + r] * Qm[3 * m + cc];     // (L^T Q)_rc
+                qiq += QI[3 * r + m] * Qm[3 * m + cc];    // ((Q - I) Q)_rc
+            }
+            float dq = ql + ltq - q.inv_relax * QI[3 * r + cc] - beta * qiq;
+            Qn[3 * r + cc] = Qm[3 * r + cc] + dq * dt;
+        }
+    const float tr = L[0] + L[4] + L[8];   // trace of D equals trace of L
+    const float shear = sqrtf(0.5f * tr * tr);
+    float mu = fminf(q.vis_base * kap, q.vis_max);
+    float s5 = 10.0f * shear;
+    s5 = s5 * s5 * s5 * s5 * s5;
+    mu = q.vis_base + (mu - q.vis_base) / (1.0f + s5) * fminf(1.0f, ap / (q.theta + 1e-6f));
+    vis[i] = mu;
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+        Q[a * st + i] = Qn[a];
+        Lout[a * st + i] = L[a];
+        stress[a * st + i] = mu * (Qn[a] - ((a == 0 || a == 4 || a == 8) ? 1.0f : 0.0f));
+    }
+}
+
+// add_phase_pct_acc (IMMCTCudaImpl.cu:746-780) + phase velocities + mixture velocity:
+//   a = a_p rest_i sum_{j fluid} (S_i/rest_i^2 + S_j/rest_j^2) gradW_ij,
+//   u_k += a (Cd + (1-Cd) rest_i/rho_k) dt.
+template <int MP>
+__global__ __launch_bounds__(256) void k_immct_stress_acc(const float4 *__restrict__ x, float4 *__restrict__ v,
+                                                          float4 *__restrict__ vadv,
+                                                          const int32_t *__restrict__ mat,
+                                                          const float *__restrict__ rest,
+                                                          const float *__restrict__ cd,
+                                                          const float *__restrict__ fpoly,
+                                                          const float *__restrict__ stress,
+                                                          const uint32_t *__restrict__ nbr,
+                                                          const uint32_t *__restrict__ cnt, uint32_t n,
+                                                          uint32_t stride, ImmK q, Ph ph, const SolverScalars *sc) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (mat[i] != VTP_MAT_FLUID) return;
+    const float dt = sc->dt;
+    const float4 xi = x[i];
+    const float ri = rest[i];
+    const float ir2 = 1.0f / (ri * ri);
+    const size_t st = stride;
+    float Si[9];
+#pragma unroll
+    for (int a = 0; a < 9; ++a) Si[a] = stress[a * st + i] * ir2;
+    const uint32_t nn = cnt[i] & ~kDeadBit;
+    float ax = 0, ay = 0, az = 0;
+    const uint32_t *p = nbr + i;
+    for (uint32_t t = 0; t < nn; ++t) {
+        uint32_t j = p[(size_t)t * stride];
+        if (mat[j] != VTP_MAT_FLUID) continue;
+        float4 xj = x[j];
+        float rj = rest[j];
+        float jr2 = 1.0f / (rj * rj);
+        float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        float f = cubic_grad_factor(dx * dx + dy * dy + dz * dz, q.k);
+        if (q.mass_weighted) f *= xj.w;
+        float gx = dx * f, gy = dy * f, gz = dz * f;
+        float T[9];
+#pragma unroll
+        for (int a = 0; a < 9; ++a) T[a] = Si[a] + stress[a * st + j] * jr2;
+        ax += T[0] * gx + T[1] * gy + T[2] * gz;
+        ay += T[3] * gx + T[4] * gy + T[5] * gz;
+        az += T[6] * gx + T[7] * gy + T[8] * gz;
+    }
+    const float sc_a = fpoly[i] * ri;
+    ax *= sc_a;
+    ay *= sc_a;
+    az *= sc_a;
+    const float ci = cd[i];
+#pragma unroll
+    for (int k = 0; k < MP; ++k) {
+        if (k >= q.P) break;
+        float s = (ci + (1.0f - ci) * (ri / q.rho[k])) * dt;
+        size_t o = (size_t)k * ph.stride + i;
+        float4 u = ph.u[o];
+        u.x += ax * s;
+        u.y += ay * s;
+        u.z += az * s;
+        ph.u[o] = u;
+    }
+    v[i] = mix_velocity<MP>(ph, i, q.P, v[i], vadv);
+}
+
+// ------------------------------------------------------------------ phase transfer
+// update_phase_change_from_drift + ..._from_diffuse (IMMCudaImpl.cu:532-605) into
+// fresh in/out accumulators; flagged particles (as i or j) are skipped.
+template <int MP>
+__global__ __launch_bounds__(256) void k_imm_fluxes(const float4 *__restrict__ x, const float4 *__restrict__ v,
+                                                    const int32_t *__restrict__ mat,
+                                                    const uint32_t *__restrict__ flag,
+                                                    const uint32_t *__restrict__ nbr,
+                                                    const uint32_t *__restrict__ cnt, uint32_t n, uint32_t stride,
+                                                    ImmK q, Ph ph, const uint32_t *status, const SolverScalars *sc) {
+    if (status[kStDone]) return;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t mi = mat[i];
+    if (mi != VTP_MAT_FLUID) return;
+    float fin[MP], fout[MP], fi[MP];
+    float4 di[MP];
+#pragma unroll
+    for (int k = 0; k < MP; ++k) {
+        fin[k] = 0.0f;
+        fout[k] = 0.0f;
+        fi[k] = k < q.P ? ph.frac[(size_t)k * ph.stride + i] : 0.0f;
+        di[k] = k < q.P ? ph.drift[(size_t)k * ph.stride + i] : f4(0, 0, 0);
+    }
+    if (!flag[i]) {
+        const float dt = sc->dt;
+        const float4 xi = x[i];
+        const uint32_t nn = cnt[i] & ~kDeadBit;
+        const uint32_t *p = nbr + i;
+        for (uint32_t t = 0; t < nn; ++t) {
+            uint32_t j = p[(size_t)t * stride];
+            if (mat[j] != mi || flag[j]) continue;
+            float4 xj = x[j];
+            float Vj = v[j].w;
+            float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            float f = cubic_grad_factor(r2, q.k);
+            float gx = dx * f, gy = dy * f, gz = dz * f;
+            float factor = f;   // (gradW . x_ij) / |x_ij|^2 = f
+#pragma unroll
+            for (int k = 0; k < MP; ++k) {
+                if (k >= q.P) break;
+                size_t o = (size_t)k * ph.stride + j;
+                float fj = ph.frac[o];
+                float4 dj = ph.drift[o];
+                float c1 = -dt * Vj * ((fi[k] * di[k].x + fj * dj.x) * gx + (fi[k] * di[k].y + fj * dj.y) * gy +
+                                       (fi[k] * di[k].z + fj * dj.z) * gz);
+                if (c1 < 0.0f) fout[k] += c1; else fin[k] += c1;
+                float c2 = dt * q.cf * (fi[k] - fj) * Vj * factor;
+                if (c2 < 0.0f) fout[k] += c2; else fin[k] += c2;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < MP; ++k) {
+        if (k >= q.P) break;
+        ph.fin[(size_t)k * ph.stride + i] = fin[k];
+        ph.fout[(size_t)k * ph.stride + i] = fout[k];
+    }
+}
+
+__global__ void k_phase_begin(uint32_t *status) {
+    if (threadIdx.x == 0) {
+        status[kStDone] = 0;
+        status[kStIter] = 1;
+        status[kStNegCount] = 0;
+    }
+}
+
+// check_negative_cuda (IMMCudaImpl.cu:607-633): flag unflagged fluid particles
+// whose fraction would turn negative; count them (counter reset by the
+// decide kernel, race-free).
+template <int MP>
+__global__ void k_imm_check_negative(const int32_t *__restrict__ mat, uint32_t *__restrict__ flag, uint32_t n,
+                                     ImmK q, Ph ph, uint32_t *status) {
+    if (status[kStDone]) return;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool neg = false;
+    if (i < n && mat[i] == VTP_MAT_FLUID && !flag[i]) {
+#pragma unroll
+        for (int k = 0; k < MP; ++k) {
+            if (k >= q.P) break;
+            size_t o = (size_t)k * ph.stride + i;
+            if (ph.frac[o] + ph.fout[o] + ph.fin[o] < 0.0f) neg = true;
+        }
+        if (neg) flag[i] = 1;
+    }
+    uint64_t b = __ballot(neg);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&status[kStNegCount], (uint32_t)__popcll(b));
+}
+
+// Loop rule of phase_transfer (:903-932): stop when nothing new was flagged or
+// iter > cap; otherwise the fluxes are recomputed and iter increments.
+__global__ void k_imm_phase_decide(uint32_t *status, uint32_t cap) {
+    if (threadIdx.x != 0 || status[kStDone]) return;
+    uint32_t iter = status[kStIter];
+    uint32_t neg = status[kStNegCount];
+    status[kStNegCount] = 0;
+    if (neg == 0 || iter > cap) {
+        status[kStDone] = 1;
+        status[kStPhaseIter] = iter;
+        if (iter > cap) atomicOr(&status[kStCrash], (uint32_t)VTP_CRASH_PHASE_ITER_CAP);
+    } else {
+        status[kStIter] = iter + 1;
+    }
+}
+
+// update_phase_change + release_unused_drift_vel + release_negative +
+// regularize (x2, phase_transfer and update_mass_and_vel) + rest density and
+// mass (fluid part) + update_vel_from_phase_vel + update_color
+// (IMMCudaImpl.cu:484-515, 635-676, 934-980).
+template <int MP>
+__global__ void k_imm_phase_update(float4 *__restrict__ x, float4 *__restrict__ v, float4 *__restrict__ vadv,
+                                   const int32_t *__restrict__ mat, uint32_t *__restrict__ flag,
+                                   float *__restrict__ rest, float4 *__restrict__ color, uint32_t n, ImmK q,
+                                   Ph ph) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (mat[i] != VTP_MAT_FLUID) {
+        float r = 0.0f;
+#pragma unroll
+        for (int k = 0; k < MP; ++k)
+            if (k < q.P) r += ph.frac[(size_t)k * ph.stride + i] * q.rho[k];
+        rest[i] = r;   // own part; the fluid-neighbour maximum follows
+        return;
+    }
+    const float4 vi = v[i];
+    const bool fl = flag[i] != 0;
+    float a[MP];
+    float sum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < MP; ++k) {
+        if (k >= q.P) { a[k] = 0.0f; continue; }
+        size_t o = (size_t)k * ph.stride + i;
+        a[k] = ph.frac[o] + (ph.fout[o] + ph.fin[o]);
+        if (fl) ph.u[o] = f4(vi.x, vi.y, vi.z);
+    }
+    flag[i] = 0;
+#pragma unroll
+    for (int k = 0; k < MP; ++k) sum += a[k];
+#pragma unroll
+    for (int k = 0; k < MP; ++k) a[k] /= sum;
+    sum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < MP; ++k) sum += a[k];
+    float r = 0.0f, cx = 0, cy = 0, cz = 0;
+#pragma unroll
+    for (int k = 0; k < MP; ++k) {
+        if (k >= q.P) break;
+        a[k] /= sum;
+        ph.frac[(size_t)k * ph.stride + i] = a[k];
+        r += a[k] * q.rho[k];
+        cx += a[k] * q.col[k][0];
+        cy += a[k] * q.col[k][1];
+        cz += a[k] * q.col[k][2];
+    }
+    rest[i] = r;
+    x[i].w = r * vi.w;
+    v[i] = mix_velocity<MP>(ph, i, q.P, vi, vadv);
+    color[i] = f4(cx, cy, cz);
+}
+
+// gathers of the carried state into the new slot order
+__global__ void k_gather_f32(const float *__restrict__ src, float *__restrict__ dst, const uint32_t *__restrict__ perm,
+                             uint32_t n, uint32_t stride, uint32_t comps) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t o = perm[i];
+    for (uint32_t c = 0; c < comps; ++c) dst[(size_t)c * stride + i] = src[(size_t)c * stride + o];
+}
+__global__ void k_gather_f4(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                            const uint32_t *__restrict__ perm, uint32_t n, uint32_t stride, uint32_t comps) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t o = perm[i];
+    for (uint32_t c = 0; c < comps; ++c) dst[(size_t)c * stride + i] = src[(size_t)c * stride + o];
+}
+__global__ void k_extract_phase(const float *__restrict__ frac, float *__restrict__ out, uint32_t n, uint32_t stride,
+                                int k) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = frac[(size_t)k * stride + i];
+}
+
+// ------------------------------------------------------------------ host side
+void common_upload_and_search(Ctx &c);
+void common_reorder(Ctx &c, const float4 *xs, float4 *xd, float4 *xd2, const float4 *vs, float4 *vd, float4 *vd2);
+void launch_step_begin(Ctx &c);
+void launch_hash(Ctx &c, const float4 *x);
+void dfsph_allocate_common(Ctx &c);
+void df_pressure_solve(Ctx &c, const float4 *x, const float4 *v, int which);
+
+static Ph phases(Ctx &c) {
+    return Ph{c.frac[0].p, c.uphase[0].p, c.aphase.p, c.drift.p, c.frac_in.p, c.frac_out.p, c.ugns.stride};
+}
+
+template <int MP>
+static void imm_init_impl(Ctx &c) {
+    const uint32_t n = c.n, nb = div_up(n, 256);
+    const ImmK q = imm_constants(c);
+    Ph ph = phases(c);
+    hipLaunchKernelGGL(k_imm_init_a<MP>, dim3(nb), dim3(256), 0, c.stream, c.X[0].p, c.V[0].p, c.vadv.p,
+                       c.rest2[0].p, c.cd.p, c.vis_ct.p, c.Q[0].p, c.color4[0].p, c.mat.p, n, q, ph);
+    // prepare_imm: rest density (boundary max) and mass with V = V0, then boundary volume
+    VTP_HIP(hipMemcpyAsync(c.kappa.p, c.rest2[0].p, n * 4, hipMemcpyDeviceToDevice, c.stream));
+    hipLaunchKernelGGL(k_imm_boundary_rest, dim3(nb), dim3(256), 0, c.stream, c.X[0].p, c.V[0].p, c.kappa.p,
+                       c.rest2[0].p, c.mat.p, c.ugns.nbr.p, c.ugns.cnt.p, n, c.ugns.stride, c.kern, 1);
+    VTP_HIP(hipGetLastError());
+}
+
+void imm_initialize(Ctx &c) {
+    const uint32_t n = c.n;
+    const uint32_t P = c.p.phase_num;
+    if (P == 0 || P > VTP_MAX_PHASES) throw std::runtime_error("IMM needs 1..8 phases");
+    c.P = P;
+    dfsph_allocate_common(c);
+    common_upload_and_search(c);
+    const uint32_t st = c.ugns.stride;
+    for (int b = 0; b < 2; ++b) {
+        c.frac[b].alloc((size_t)P * st);
+        c.uphase[b].alloc((size_t)P * st);
+        c.color4[b].alloc(st);
+        c.rest2[b].alloc(st);
+        if (c.p.solver == VTP_IMMCT) c.Q[b].alloc(9 * (size_t)st);
+    }
+    c.aphase.alloc((size_t)P * st);
+    c.drift.alloc((size_t)P * st);
+    c.frac_in.alloc((size_t)P * st);
+    c.frac_out.alloc((size_t)P * st);
+    c.flag.alloc(st);
+    c.cd.alloc(st);
+    c.fpoly.alloc(st);
+    c.vis_ct.alloc(st);
+    c.Lgrad.alloc(9 * (size_t)st);
+    c.stress.alloc(9 * (size_t)st);
+    if (c.p.solver != VTP_IMMCT) c.Q[0].alloc(9);
+    VTP_HIP(hipMemsetAsync(c.flag.p, 0, c.flag.bytes(), c.stream));
+    VTP_HIP(hipMemsetAsync(c.Lgrad.p, 0, c.Lgrad.bytes(), c.stream));
+    VTP_HIP(hipMemsetAsync(c.stress.p, 0, c.stress.bytes(), c.stream));
+    VTP_HIP(hipMemsetAsync(c.aphase.p, 0, c.aphase.bytes(), c.stream));
+    // attach-order fractions -> phase-major slot order
+    std::vector<uint32_t> orig(n);
+    VTP_HIP(hipMemcpyAsync(orig.data(), c.orig.p, n * 4, hipMemcpyDeviceToHost, c.stream));
+    VTP_HIP(hipStreamSynchronize(c.stream));
+    std::vector<float> fr((size_t)P * st, 0.0f);
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t k = 0; k < P; ++k) fr[(size_t)k * st + i] = c.h_frac[(size_t)orig[i] * P + k];
+    VTP_HIP(hipMemcpyAsync(c.frac[0].p, fr.data(), fr.size() * 4, hipMemcpyHostToDevice, c.stream));
+    // rest density and colour live in the ping-pong state buffers rest2 / color4
+    if (P <= 4) imm_init_impl<4>(c); else imm_init_impl<8>(c);
+    VTP_HIP(hipStreamSynchronize(c.stream));
+}
+
+template <int MP>
+static void imm_step_impl(Ctx &c) {
+    const uint32_t n = c.n, nb = div_up(n, 256);
+    const ImmK q = imm_constants(c);
+    Ugns &u = c.ugns;
+    const uint32_t st = u.stride;
+    const uint32_t P = c.P;
+    launch_step_begin(c);
+    // ---- neighbour search; carried state into the new slot order
+    c.timer.begin(c.stream, "imm_hash");
+    launch_hash(c, c.X[0].p);
+    c.timer.end(c.stream);
+    u.sort_and_rank(c.orig.p, c.stream, c.timer);
+    c.timer.begin(c.stream, "imm_reorder");
+    common_reorder(c, c.X[0].p, c.X[1].p, nullptr, c.V[0].p, c.V[1].p, nullptr);
+    hipLaunchKernelGGL(k_gather_f32, dim3(nb), dim3(256), 0, c.stream, c.frac[0].p, c.frac[1].p, u.perm.p, n, st, P);
+    hipLaunchKernelGGL(k_gather_f4, dim3(nb), dim3(256), 0, c.stream, c.uphase[0].p, c.uphase[1].p, u.perm.p, n, st,
+                       P);
+    hipLaunchKernelGGL(k_gather_f32, dim3(nb), dim3(256), 0, c.stream, c.rest2[0].p, c.rest2[1].p, u.perm.p, n, st, 1u);
+    hipLaunchKernelGGL(k_gather_f4, dim3(nb), dim3(256), 0, c.stream, c.color4[0].p, c.color4[1].p, u.perm.p, n, st,
+                       1u);
+    if (q.ct)
+        hipLaunchKernelGGL(k_gather_f32, dim3(nb), dim3(256), 0, c.stream, c.Q[0].p, c.Q[1].p, u.perm.p, n, st, 9u);
+    c.timer.end(c.stream);
+    std::swap(c.X[0].p, c.X[1].p);
+    std::swap(c.V[0].p, c.V[1].p);
+    std::swap(c.frac[0].p, c.frac[1].p);
+    std::swap(c.uphase[0].p, c.uphase[1].p);
+    std::swap(c.rest2[0].p, c.rest2[1].p);
+    std::swap(c.color4[0].p, c.color4[1].p);
+    if (q.ct) std::swap(c.Q[0].p, c.Q[1].p);
+    float4 *x = c.X[0].p, *v = c.V[0].p;
+    float *rest = c.rest2[0].p;
+    Ph ph = phases(c);
+    u.build_neighbors(x, c.stream, c.timer);
+    if (q.ct)
+        hipLaunchKernelGGL(k_extract_phase, dim3(nb), dim3(256), 0, c.stream, ph.frac, c.fpoly.p, n, st, q.pp);
+    const uint32_t B = c.block("imm_pass");
+    const uint32_t nbB = div_up(n, B);
+    c.timer.begin(c.stream, "imm_precompute");
+    hipLaunchKernelGGL(k_imm_precompute, dim3(nbB), dim3(B), 0, c.stream, x, v, c.vadv.p, c.mat.p, u.nbr.p, u.cnt.p, n,
+                       st, q, c.fpoly.p, c.cr.p, c.alpha.p, c.cd.p);
+    c.timer.end(c.stream);
+    df_pressure_solve(c, x, v, 0);
+    c.timer.begin(c.stream, "imm_pressure_to_phases");
+    hipLaunchKernelGGL(k_imm_pressure_to_phases<MP>, dim3(nb), dim3(256), 0, c.stream, x, v, c.vadv.p, c.acc.p,
+                       c.mat.p, rest, c.cd.p, n, q, ph, c.scal.p, 0);
+    c.timer.end(c.stream);
+    c.timer.begin(c.stream, "imm_vis_normal");
+    hipLaunchKernelGGL(k_imm_vis_normal<MP>, dim3(nbB), dim3(B), 0, c.stream, x, v, c.mat.p, rest, c.cd.p, u.nbr.p,
+                       u.cnt.p, n, st, q, ph, c.normal.p);
+    c.timer.end(c.stream);
+    c.timer.begin(c.stream, "imm_tension_phase_vel");
+    hipLaunchKernelGGL(k_imm_tension_phase_vel<MP>, dim3(nbB), dim3(B), 0, c.stream, x, v, c.vadv.p, c.mat.p, rest,
+                       c.normal.p, u.nbr.p, u.cnt.p, n, st, q, ph, c.scal.p);
+    c.timer.end(c.stream);
+    if (q.ct) {
+        c.timer.begin(c.stream, "immct_conformation");
+        hipLaunchKernelGGL(k_immct_conformation, dim3(nbB), dim3(B), 0, c.stream, x, v, c.mat.p, rest, u.nbr.p,
+                           u.cnt.p, n, st, q, c.fpoly.p, c.Q[0].p, c.Lgrad.p, c.stress.p, c.vis_ct.p, c.scal.p);
+        c.timer.end(c.stream);
+        c.timer.begin(c.stream, "immct_stress_acc");
+        hipLaunchKernelGGL(k_immct_stress_acc<MP>, dim3(nbB), dim3(B), 0, c.stream, x, v, c.vadv.p, c.mat.p, rest,
+                           c.cd.p, c.fpoly.p, c.stress.p, u.nbr.p, u.cnt.p, n, st, q, ph, c.scal.p);
+        c.timer.end(c.stream);
+    }
+    df_pressure_solve(c, x, v, 1);
+    c.timer.begin(c.stream, "imm_pressure_update_pos");
+    hipLaunchKernelGGL(k_imm_pressure_to_phases<MP>, dim3(nb), dim3(256), 0, c.stream, x, v, c.vadv.p, c.acc.p,
+                       c.mat.p, rest, c.cd.p, n, q, ph, c.scal.p, 1);
+    c.timer.end(c.stream);
+    // ---- phase transfer with the negative-fraction retry loop
+    const uint32_t cap = c.p.max_solver_iterations;
+    hipLaunchKernelGGL(k_phase_begin, dim3(1), dim3(64), 0, c.stream, c.status.p);
+    c.timer.begin(c.stream, "imm_fluxes");
+    hipLaunchKernelGGL(k_imm_fluxes<MP>, dim3(nbB), dim3(B), 0, c.stream, x, v, c.mat.p, c.flag.p, u.nbr.p, u.cnt.p,
+                       n, st, q, ph, c.status.p, c.scal.p);
+    c.timer.end(c.stream);
+    uint32_t chunk = 2, launched = 0;
+    for (;;) {
+        for (uint32_t it = 0; it < chunk && launched <= cap; ++it, ++launched) {
+            hipLaunchKernelGGL(k_imm_check_negative<MP>, dim3(nb), dim3(256), 0, c.stream, c.mat.p, c.flag.p, n, q,
+                               ph, c.status.p);
+            hipLaunchKernelGGL(k_imm_phase_decide, dim3(1), dim3(64), 0, c.stream, c.status.p, cap);
+            hipLaunchKernelGGL(k_imm_fluxes<MP>, dim3(nbB), dim3(B), 0, c.stream, x, v, c.mat.p, c.flag.p, u.nbr.p,
+                               u.cnt.p, n, st, q, ph, c.status.p, c.scal.p);
+        }
+        read_status(c);
+        if (c.h_status[kStDone] || launched > cap) break;
+        chunk = std::min(chunk * 2, cap + 1);
+    }
+    c.timer.begin(c.stream, "imm_phase_update");
+    hipLaunchKernelGGL(k_imm_phase_update<MP>, dim3(nb), dim3(256), 0, c.stream, x, v, c.vadv.p, c.mat.p, c.flag.p,
+                       rest, c.color4[0].p, n, q, ph);
+    VTP_HIP(hipMemcpyAsync(c.kappa.p, rest, n * 4, hipMemcpyDeviceToDevice, c.stream));
+    hipLaunchKernelGGL(k_imm_boundary_rest, dim3(nb), dim3(256), 0, c.stream, x, v, c.kappa.p, rest, c.mat.p,
+                       u.nbr.p, u.cnt.p, n, st, c.kern, 0);
+    c.timer.end(c.stream);
+    VTP_HIP(hipGetLastError());
+}
+
+void imm_step(Ctx &c) {
+    if (c.P <= 4) imm_step_impl<4>(c); else imm_step_impl<8>(c);
+}
+
 }  // namespace vtp

@@ -16,15 +16,7 @@
 
 namespace vtp {
 
-struct PbfK {
-    Cubic k;
-    float dt, inv_dt;
-    float rho0, inv_rho0;
-    float xsph_k;
-    float gx, gy, gz;
-};
-
-static PbfK pbf_constants(const Ctx &c) {
+PbfK pbf_constants(const Ctx &c) {
     PbfK q;
     q.k = c.kern;
     q.dt = c.p.time_step;
@@ -35,6 +27,7 @@ static PbfK pbf_constants(const Ctx &c) {
     q.gx = c.p.gravity[0];
     q.gy = c.p.gravity[1];
     q.gz = c.p.gravity[2];
+    q.mass_weighted = c.p.mass_weighted;
     return q;
 }
 
@@ -78,7 +71,8 @@ __global__ __launch_bounds__(256) void k_pbf_lambda(const float4 *__restrict__ x
                                                     const uint32_t *__restrict__ nbr,
                                                     const uint32_t *__restrict__ cnt, uint32_t n,
                                                     uint32_t stride, PbfK q, float *__restrict__ lam,
-                                                    float *__restrict__ dens, float *__restrict__ errc) {
+                                                    float *__restrict__ dens, float *__restrict__ errc,
+                                                    float4 *__restrict__ sg) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (mat[i] != VTP_MAT_FLUID) {
@@ -86,6 +80,7 @@ __global__ __launch_bounds__(256) void k_pbf_lambda(const float4 *__restrict__ x
         lam[i] = 0.0f;
         dens[i] = 0.0f;
         errc[i] = 0.0f;
+        if (sg) sg[i] = make_float4(0, 0, 0, 0);
         return;
     }
     const float4 xi = x[i];
@@ -93,20 +88,21 @@ __global__ __launch_bounds__(256) void k_pbf_lambda(const float4 *__restrict__ x
     const uint32_t nn = w & ~kDeadBit;
     float rho = (w & kDeadBit) ? 0.0f : xi.w * q.k.w0;
     float gsx = 0, gsy = 0, gsz = 0, s2 = 0;
-    for (uint32_t t = 0; t < nn; ++t) {
-        uint32_t j = nbr[(size_t)t * stride + i];
-        float4 xj = x[j];
-        float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        float r2 = dx * dx + dy * dy + dz * dz;
-        float W, f;
-        cubic_w_grad(r2, q.k, W, f);
-        rho = fmaf(xj.w, W, rho);
-        float ax = dx * f, ay = dy * f, az = dz * f;
-        gsx += ax;
-        gsy += ay;
-        gsz += az;
-        s2 += ax * ax + ay * ay + az * az;
-    }
+    for_each_neighbor(
+        nbr, stride, i, nn, [&](uint32_t j) { return x[j]; },
+        [&](const float4 &xj) {
+            float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            float W, f;
+            cubic_w_grad(r2, q.k, W, f);
+            rho = fmaf(xj.w, W, rho);
+            if (q.mass_weighted) f *= xj.w;
+            float ax = dx * f, ay = dy * f, az = dz * f;
+            gsx += ax;
+            gsy += ay;
+            gsz += az;
+            s2 += ax * ax + ay * ay + az * az;
+        });
     rho = fmaxf(rho, q.rho0);
     float C = rho * q.inv_rho0 - 1.0f;
     float ir2 = q.inv_rho0 * q.inv_rho0;
@@ -114,6 +110,7 @@ __global__ __launch_bounds__(256) void k_pbf_lambda(const float4 *__restrict__ x
     lam[i] = -C / (S + 1e-6f);
     dens[i] = rho;
     errc[i] = C;
+    if (sg) sg[i] = make_float4(gsx, gsy, gsz, S);   // taped for the adjoint
 }
 
 // compute_dx_cuda + apply_dx_cuda (PBFCudaImpl.cu:124-170):
@@ -131,17 +128,20 @@ __global__ __launch_bounds__(256) void k_pbf_dx(const float4 *__restrict__ x, fl
     const float li = lam[i];
     const uint32_t nn = cnt[i] & ~kDeadBit;
     float ax = 0, ay = 0, az = 0;
-    for (uint32_t t = 0; t < nn; ++t) {
-        uint32_t j = nbr[(size_t)t * stride + i];
-        float4 xj = x[j];
-        float lj = lam[j];
-        float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        float r2 = dx * dx + dy * dy + dz * dz;
-        float s = cubic_grad_factor(r2, q.k) * (li + lj);
-        ax = fmaf(dx, s, ax);
-        ay = fmaf(dy, s, ay);
-        az = fmaf(dz, s, az);
-    }
+    for_each_neighbor(
+        nbr, stride, i, nn,
+        [&](uint32_t j) {
+            float4 xj = x[j];
+            return make_float4(xj.x, xj.y, xj.z, (li + lam[j]) * (q.mass_weighted ? xj.w : 1.0f));
+        },
+        [&](const float4 &xj) {
+            float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            float s = cubic_grad_factor(r2, q.k) * xj.w;
+            ax = fmaf(dx, s, ax);
+            ay = fmaf(dy, s, ay);
+            az = fmaf(dz, s, az);
+        });
     ax *= q.inv_rho0;
     ay *= q.inv_rho0;
     az *= q.inv_rho0;
@@ -175,17 +175,17 @@ __global__ __launch_bounds__(256) void k_pbf_xsph(const float4 *__restrict__ x, 
     }
     const uint32_t nn = cnt[i] & ~kDeadBit;
     float ax = 0, ay = 0, az = 0;
-    for (uint32_t t = 0; t < nn; ++t) {
-        uint32_t j = nbr[(size_t)t * stride + i];
-        float4 xj = x[j];
-        float4 vj = v[j];
-        float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        float r2 = dx * dx + dy * dy + dz * dz;
-        float s = vj.w * cubic_w(r2, q.k);
-        ax = fmaf(vj.x - vi.x, s, ax);
-        ay = fmaf(vj.y - vi.y, s, ay);
-        az = fmaf(vj.z - vi.z, s, az);
-    }
+    struct XV { float4 x, v; };
+    for_each_neighbor(
+        nbr, stride, i, nn, [&](uint32_t j) { return XV{x[j], v[j]}; },
+        [&](const XV &d) {
+            float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            float s = d.v.w * cubic_w(r2, q.k);
+            ax = fmaf(d.v.x - vi.x, s, ax);
+            ay = fmaf(d.v.y - vi.y, s, ay);
+            az = fmaf(d.v.z - vi.z, s, az);
+        });
     float c = q.xsph_k * q.dt;
     float dx = ax * c, dy = ay * c, dz = az * c;
     xo[i] = make_float4(xi.x + dx, xi.y + dy, xi.z + dz, xi.w);
@@ -207,7 +207,8 @@ __global__ void k_pbf_init(float4 *__restrict__ x, float4 *__restrict__ v, const
         const uint32_t w = cnt[i];
         const uint32_t nn = w & ~kDeadBit;
         float delta = (w & kDeadBit) ? 0.0f : k.w0;
-        for (uint32_t t = 0; t < nn; ++t) {
+    #pragma unroll 4
+    for (uint32_t t = 0; t < nn; ++t) {
             uint32_t j = nbr[(size_t)t * stride + i];
             if (mat[j] != mi) continue;
             float4 xj = x[j];
@@ -242,7 +243,7 @@ void pbf_initialize(Ctx &c) {
     VTP_HIP(hipGetLastError());
 }
 
-static void pbf_enqueue_step(Ctx &c) {
+static void pbf_enqueue_step(Ctx &c, PbfTapeStep *tape = nullptr) {
     const uint32_t n = c.n;
     const PbfK q = pbf_constants(c);
     Ugns &u = c.ugns;
@@ -257,23 +258,44 @@ static void pbf_enqueue_step(Ctx &c) {
     common_reorder(c, c.X[0].p, c.X[1].p, c.X[2].p, c.V[0].p, c.V[1].p, nullptr);
     c.timer.end(c.stream);
     u.build_neighbors(c.X[1].p, c.stream, c.timer);
+    if (tape) {
+        VTP_HIP(hipMemcpyAsync(tape->perm, u.perm.p, n * 4, hipMemcpyDeviceToDevice, c.stream));
+        VTP_HIP(hipMemcpyAsync(tape->orig, c.orig.p, n * 4, hipMemcpyDeviceToDevice, c.stream));
+        VTP_HIP(hipMemcpyAsync(tape->cnt, u.cnt.p, n * 4, hipMemcpyDeviceToDevice, c.stream));
+        VTP_HIP(hipMemcpyAsync(tape->nbr, u.nbr.p, (size_t)u.cap * u.stride * 4, hipMemcpyDeviceToDevice,
+                               c.stream));
+    }
     float4 *cur = c.X[1].p, *nxt = c.X[2].p;
     const uint32_t bl = c.block("pbf_lambda"), bd = c.block("pbf_dx"), bx = c.block("pbf_xsph");
     for (uint32_t it = 0; it < c.p.iteration_num; ++it) {
         c.timer.begin(c.stream, "pbf_lambda");
+        float *lam = tape ? tape->lam[it] : c.lam.p;
+        float *errc = tape ? tape->C[it] : c.err_c.p;
+        if (tape) VTP_HIP(hipMemcpyAsync(tape->x[it], cur, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
         hipLaunchKernelGGL(k_pbf_lambda, dim3(div_up(n, bl)), dim3(bl), 0, c.stream, cur, c.mat.p, u.nbr.p,
-                           u.cnt.p, n, u.stride, q, c.lam.p, c.dens.p, c.err_c.p);
+                           u.cnt.p, n, u.stride, q, lam, c.dens.p, errc, tape ? tape->sg[it] : nullptr);
         c.timer.end(c.stream);
         c.timer.begin(c.stream, "pbf_dx");
         hipLaunchKernelGGL(k_pbf_dx, dim3(div_up(n, bd)), dim3(bd), 0, c.stream, cur, nxt, c.V[1].p, c.mat.p,
-                           u.nbr.p, u.cnt.p, n, u.stride, q, c.lam.p);
+                           u.nbr.p, u.cnt.p, n, u.stride, q, lam);
         c.timer.end(c.stream);
         std::swap(cur, nxt);
+    }
+    if (tape) {
+        const uint32_t K = c.p.iteration_num;
+        VTP_HIP(hipMemcpyAsync(tape->x[K], cur, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
+        VTP_HIP(hipMemcpyAsync(tape->v_last, c.V[1].p, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
     }
     c.timer.begin(c.stream, "pbf_xsph");
     hipLaunchKernelGGL(k_pbf_xsph, dim3(div_up(n, bx)), dim3(bx), 0, c.stream, cur, c.X[0].p, c.V[1].p,
                        c.V[0].p, c.mat.p, u.nbr.p, u.cnt.p, n, u.stride, q);
     c.timer.end(c.stream);
+}
+
+// One tick that also records what the reverse pass needs (no graph).
+void pbf_step_taped(Ctx &c, PbfTapeStep &tape) {
+    pbf_enqueue_step(c, &tape);
+    VTP_HIP(hipGetLastError());
 }
 
 void pbf_capture_reset(Ctx &c) {
@@ -284,6 +306,7 @@ void pbf_capture_reset(Ctx &c) {
 }
 
 void pbf_step(Ctx &c) {
+    c.lists_stale = false;
     if (c.opt_graph && !c.timer.enabled) {
         if (!c.graph_exec) {
             VTP_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));

@@ -202,32 +202,15 @@ __global__ void k_cell_fixup(const uint32_t *__restrict__ key, const uint32_t *_
     orig_new[dst] = my;
 }
 
-// Neighbour lists, UGNSCudaImpl.cu:168-203: for the 27 cells in the
-// reference's offset order (x outermost, z innermost,
-// UniformGridNeighborSearch.cpp:33-60), every other particle of the cell in
-// slot order that passes 0 < |x_i - x_j| <= h, capped at M-1 entries.
-__global__ __launch_bounds__(256) void k_build_neighbors(const float4 *__restrict__ x,
-                                                         const uint32_t *__restrict__ key,
-                                                         const uint32_t *__restrict__ start,
-                                                         const uint32_t *__restrict__ end, Grid g, uint32_t n,
-                                                         uint32_t stride, uint32_t cap,
-                                                         uint32_t *__restrict__ nbr, uint32_t *__restrict__ cnt) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t k = key[i];
-    if (k >= g.dead_key) {
-        cnt[i] = kDeadBit;
-        return;
-    }
+// Reference traversal of one particle (UGNSCudaImpl.cu:168-203): the 27
+// cells in the reference's offset order (x outermost, z innermost,
+// UniformGridNeighborSearch.cpp:33-60), every other particle of a cell in
+// slot order that passes 0 < |x_i - x_j| <= h, capped at `cap` entries.
+__device__ uint32_t reference_scan(const float4 *__restrict__ x, const uint32_t *__restrict__ start,
+                                   const uint32_t *__restrict__ end, const Grid &g, uint32_t i, float3 p, int cx,
+                                   int cy, int cz, uint32_t sbase, uint32_t stride, uint32_t cap,
+                                   uint32_t *__restrict__ nbr) {
     const uint32_t gxy = (uint32_t)(g.gx * g.gy);
-    uint32_t scene = k / g.cells;
-    uint32_t local = k - scene * g.cells;
-    int cz = (int)(local / gxy);
-    uint32_t rem = local - (uint32_t)cz * gxy;
-    int cy = (int)(rem / (uint32_t)g.gx);
-    int cx = (int)(rem - (uint32_t)cy * (uint32_t)g.gx);
-    const uint32_t sbase = scene * g.cells;
-    float3 p = xyz(x[i]);
     uint32_t c = 0;
     for (int ox = -1; ox <= 1; ++ox) {
         int X = cx + ox;
@@ -253,14 +236,86 @@ __global__ __launch_bounds__(256) void k_build_neighbors(const float4 *__restric
             }
         }
     }
+    return c;
+}
+
+// Neighbour lists.  Cells (X-1, X, X+1) of one (Y, Z) row have consecutive
+// keys, so their particles occupy one contiguous slot range: the 27 cells
+// are walked as 9 ranges with batched candidate loads.  The resulting list
+// holds the same set as the reference traversal whenever nothing is cut by
+// the cap; a particle with more than `cap` candidates that pass is rebuilt
+// with the reference traversal so the truncated set is the reference's.
+// (The order inside a list only matters for truncation; the reference-layout
+// export re-sorts every list into the reference order.)
+__global__ __launch_bounds__(256) void k_build_neighbors(const float4 *__restrict__ x,
+                                                         const uint32_t *__restrict__ key,
+                                                         const uint32_t *__restrict__ start,
+                                                         const uint32_t *__restrict__ end, Grid g, uint32_t n,
+                                                         uint32_t stride, uint32_t cap,
+                                                         uint32_t *__restrict__ nbr, uint32_t *__restrict__ cnt) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t k = key[i];
+    if (k >= g.dead_key) {
+        cnt[i] = kDeadBit;
+        return;
+    }
+    const uint32_t gxy = (uint32_t)(g.gx * g.gy);
+    uint32_t scene = k / g.cells;
+    uint32_t local = k - scene * g.cells;
+    int cz = (int)(local / gxy);
+    uint32_t rem = local - (uint32_t)cz * gxy;
+    int cy = (int)(rem / (uint32_t)g.gx);
+    int cx = (int)(rem - (uint32_t)cy * (uint32_t)g.gx);
+    const uint32_t sbase = scene * g.cells;
+    const float3 p = xyz(x[i]);
+    const int X0 = max(cx - 1, 0), X1 = min(cx + 1, g.gx - 1);
+    uint32_t c = 0;
+    auto test = [&](uint32_t j, const float4 &xj) {
+        float d2 = dist2_exact(p, xyz(xj));
+        if (j != i && is_neighbor(d2, g.nbr_d2_max)) {
+            if (c < cap) nbr[(size_t)c * stride + i] = j;
+            ++c;
+        }
+    };
+    for (int oy = -1; oy <= 1; ++oy) {
+        int Y = cy + oy;
+        if (Y < 0 || Y >= g.gy) continue;
+        for (int oz = -1; oz <= 1; ++oz) {
+            int Z = cz + oz;
+            if (Z < 0 || Z >= g.gz) continue;
+            const uint32_t row = sbase + (uint32_t)Z * gxy + (uint32_t)Y * (uint32_t)g.gx;
+            uint32_t s = kEmpty, e = 0;
+            for (int X = X0; X <= X1; ++X) {
+                uint32_t a = start[row + X];
+                if (a != kEmpty) {
+                    if (s == kEmpty) s = a;
+                    e = end[row + X];
+                }
+            }
+            if (s == kEmpty) continue;
+            uint32_t j = s;
+            for (; j + 4 <= e + 1; j += 4) {
+                float4 a0 = x[j], a1 = x[j + 1], a2 = x[j + 2], a3 = x[j + 3];
+                test(j, a0);
+                test(j + 1, a1);
+                test(j + 2, a2);
+                test(j + 3, a3);
+            }
+            for (; j <= e; ++j) test(j, x[j]);
+        }
+    }
+    if (c > cap) c = reference_scan(x, start, end, g, i, p, cx, cy, cz, sbase, stride, cap, nbr);
     cnt[i] = c;
 }
 
 // Reference-layout export (UniformGridNeighborSearch.hpp:28-63): attach ids,
-// self first, UINT32_MAX padding.
+// self first, UINT32_MAX padding, neighbours in the reference traversal
+// order (offset t of the 27-cell table, then slot).
 __global__ void k_export_neighbors(const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ cnt,
-                                   const uint32_t *__restrict__ orig, uint32_t n, uint32_t stride, uint32_t M,
-                                   uint32_t *__restrict__ out_lists, uint32_t *__restrict__ out_num) {
+                                   const uint32_t *__restrict__ orig, const uint32_t *__restrict__ key, Grid g,
+                                   uint32_t n, uint32_t stride, uint32_t M, uint32_t *__restrict__ out_lists,
+                                   uint32_t *__restrict__ out_num) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t p = orig[i];
@@ -271,9 +326,37 @@ __global__ void k_export_neighbors(const uint32_t *__restrict__ nbr, const uint3
         for (uint32_t t = 0; t < M; ++t) row[t] = kEmpty;
         return;
     }
-    uint32_t c = w & ~kDeadBit;
+    const uint32_t c = w & ~kDeadBit;
+    const uint32_t gxy = (uint32_t)(g.gx * g.gy);
+    auto coords = [&](uint32_t kk, int &X, int &Y, int &Z) {
+        uint32_t local = kk % g.cells;
+        Z = (int)(local / gxy);
+        uint32_t rem = local - (uint32_t)Z * gxy;
+        Y = (int)(rem / (uint32_t)g.gx);
+        X = (int)(rem - (uint32_t)Y * (uint32_t)g.gx);
+    };
+    int xi, yi, zi;
+    coords(key[i], xi, yi, zi);
+    // order key: offset index t = (ox+1)*9 + (oy+1)*3 + (oz+1), then slot; insertion sort
+    auto okey = [&](uint32_t j) -> uint64_t {
+        int X, Y, Z;
+        coords(key[j], X, Y, Z);
+        uint32_t t = (uint32_t)((X - xi + 1) * 9 + (Y - yi + 1) * 3 + (Z - zi + 1));
+        return ((uint64_t)t << 32) | j;
+    };
     row[0] = p;
-    for (uint32_t t = 0; t < c; ++t) row[1 + t] = orig[nbr[(size_t)t * stride + i]];
+    for (uint32_t a = 0; a < c; ++a) {
+        uint32_t ja = nbr[(size_t)a * stride + i];
+        uint64_t ka = okey(ja);
+        uint32_t b = a;
+        // row[1..a] holds slot ids sorted so far (translated below)
+        while (b > 0 && okey(row[b]) > ka) {
+            row[b + 1] = row[b];
+            --b;
+        }
+        row[b + 1] = ja;
+    }
+    for (uint32_t t = 1; t <= c; ++t) row[t] = orig[row[t]];
     for (uint32_t t = c + 1; t < M; ++t) row[t] = kEmpty;
     out_num[p] = c + 1;
 }
@@ -342,8 +425,8 @@ void Ugns::build_neighbors(const float4 *x, hipStream_t s, StageTimer &tm) {
 }
 
 void Ugns::export_reference(const uint32_t *orig, uint32_t *d_lists, uint32_t *d_num, hipStream_t s) {
-    hipLaunchKernelGGL(k_export_neighbors, dim3(blocks_for(n, 256)), dim3(256), 0, s, nbr.p, cnt.p, orig, n,
-                       stride, M, d_lists, d_num);
+    hipLaunchKernelGGL(k_export_neighbors, dim3(blocks_for(n, 256)), dim3(256), 0, s, nbr.p, cnt.p, orig,
+                       key[sorted].p, g, n, stride, M, d_lists, d_num);
 }
 
 void launch_fill_u32(uint32_t *a, uint32_t n, uint32_t v, hipStream_t s) {

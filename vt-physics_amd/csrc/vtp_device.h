@@ -88,11 +88,15 @@ __host__ inline Cubic make_cubic(float h) {
 }
 
 // W(r) from r^2.  q <= 0.5: sigma(6q^3 - 6q^2 + 1); q <= 1: 2 sigma (1-q)^3.
+// Both branches are evaluated and selected (no divergent branch in the
+// neighbour loops).
 __device__ __forceinline__ float cubic_w(float r2, const Cubic &k) {
     float r = __fsqrt_rn(r2);
     float q = r * k.inv_h;
     float t = 1.0f - q;
-    float w = (q <= 0.5f) ? fmaf(6.0f * q * q, q - 1.0f, 1.0f) : 2.0f * t * t * t;
+    float wi = fmaf(6.0f * q * q, q - 1.0f, 1.0f);
+    float wo = 2.0f * t * t * t;
+    float w = (q <= 0.5f) ? wi : wo;
     return q <= 1.0f ? k.sigma * w : 0.0f;
 }
 
@@ -103,12 +107,14 @@ __device__ __forceinline__ float cubic_grad_factor(float r2, const Cubic &k) {
     float r = r2 * rinv;
     float q = r * k.inv_h;
     float t = 1.0f - q;
-    float g = (q <= 0.5f) ? 6.0f * q * (3.0f * q - 2.0f) : -6.0f * t * t;
+    float gi = 6.0f * q * fmaf(3.0f, q, -2.0f);
+    float go = -6.0f * t * t;
+    float g = (q <= 0.5f) ? gi : go;
     float f = k.grad_scale * g * rinv;
     return (q >= 1e-6f && q <= 1.0f) ? f : 0.0f;  // NaN (r2 == 0) -> 0
 }
 
-// W and the gradient factor together (one sqrt).
+// W and the gradient factor together (one rsqrt).
 __device__ __forceinline__ void cubic_w_grad(float r2, const Cubic &k, float &w, float &f) {
     float rinv = __frsqrt_rn(r2);
     float r = r2 * rinv;
@@ -117,8 +123,11 @@ __device__ __forceinline__ void cubic_w_grad(float r2, const Cubic &k, float &w,
     bool inner = q <= 0.5f;
     float wi = fmaf(6.0f * q * q, q - 1.0f, 1.0f);
     float wo = 2.0f * t * t * t;
-    w = q <= 1.0f ? k.sigma * (inner ? wi : wo) : 0.0f;
-    float g = inner ? 6.0f * q * (3.0f * q - 2.0f) : -6.0f * t * t;
+    float ww = inner ? wi : wo;
+    w = q <= 1.0f ? k.sigma * ww : 0.0f;
+    float gi = 6.0f * q * fmaf(3.0f, q, -2.0f);
+    float go = -6.0f * t * t;
+    float g = inner ? gi : go;
     f = (q >= 1e-6f && q <= 1.0f) ? k.grad_scale * g * rinv : 0.0f;
 }
 
@@ -129,6 +138,33 @@ __device__ __forceinline__ float surface_tension_c(float r, float h, float c_sig
     if (r * 2.0f > h && r <= h) return c_sigma * v;
     if (r > 0.0f && r * 2.0f <= h) return c_sigma * (2.0f * v - h6_64);
     return 0.0f;
+}
+
+// ---- neighbour iteration ---------------------------------------------------
+// Walks the ELL list of slot i in batches of four: the four indices are
+// loaded first, then the four gathers are issued, then the four bodies run,
+// so each lane keeps several independent loads in flight instead of one
+// index -> gather dependency chain per neighbour.
+template <typename Load, typename Use>
+__device__ __forceinline__ void for_each_neighbor(const uint32_t *__restrict__ nbr, uint32_t stride, uint32_t i,
+                                                  uint32_t nn, Load &&load, Use &&use) {
+    const uint32_t *p = nbr + i;
+    uint32_t t = 0;
+    for (; t + 4 <= nn; t += 4) {
+        const uint32_t j0 = p[(size_t)t * stride];
+        const uint32_t j1 = p[(size_t)(t + 1) * stride];
+        const uint32_t j2 = p[(size_t)(t + 2) * stride];
+        const uint32_t j3 = p[(size_t)(t + 3) * stride];
+        auto d0 = load(j0);
+        auto d1 = load(j1);
+        auto d2 = load(j2);
+        auto d3 = load(j3);
+        use(d0);
+        use(d1);
+        use(d2);
+        use(d3);
+    }
+    for (; t < nn; ++t) use(load(p[(size_t)t * stride]));
 }
 
 // ---- wavefront reductions (wave64) ----------------------------------------

@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -153,7 +154,28 @@ struct Object {
     int32_t mat;
 };
 
+// PBF constants passed by value to the kernels.
+struct PbfK {
+    Cubic k;
+    float dt, inv_dt;
+    float rho0, inv_rho0;
+    float xsph_k;
+    float gx, gy, gz;
+    int mass_weighted;   // extension: m_j in the constraint gradient
+};
+
+// Device buffers one taped PBF tick writes (owned by the adjoint state).
+struct PbfTapeStep {
+    uint32_t *perm, *orig, *cnt, *nbr;
+    float4 *x[17];       // positions entering iteration m (m < K) and after the last one (m = K)
+    float *lam[16], *C[16];
+    float4 *sg[16];      // (sum_j gradW_ij, S_i) per iteration
+    float4 *v_last;      // velocities entering XSPH
+};
+
 struct Ctx;
+PbfK pbf_constants(const Ctx &c);
+void pbf_step_taped(Ctx &c, PbfTapeStep &tape);
 void pbf_initialize(Ctx &c);
 void pbf_step(Ctx &c);
 void pbf_capture_reset(Ctx &c);
@@ -214,9 +236,9 @@ struct Ctx {
     DevBuf<float> lam, dens, err_c;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    // PBF adjoint state
-    struct Adjoint;
-    Adjoint *adj = nullptr;
+    // PBF adjoint state (checkpoints and tapes), owned here
+    std::shared_ptr<void> adj_state;
+    bool lists_stale = false;   // neighbour lists do not match X[0] (after a rollout backward)
 
     // DFSPH / IMM fields (slot order)
     DevBuf<float> rest_density, vis, cr, alpha, delta, kappa;
@@ -235,7 +257,9 @@ struct Ctx {
     DevBuf<float4> color;
     DevBuf<uint32_t> obj_of;   // object index per slot
     DevBuf<uint32_t> obj_attach;
-    DevBuf<float> cd, blocking;
+    DevBuf<float> cd, blocking, fpoly;
+    DevBuf<float4> color4[2];    // IMM colour state (ping-pong)
+    DevBuf<float> rest2[2];      // IMM rest-density state (ping-pong)
     DevBuf<float> Q[2];          // 9 per particle, [c*stride + slot]
     DevBuf<float> Lgrad, stress; // 9 per particle
     DevBuf<float> vis_ct, rheo_beta, rheo_kappa;
@@ -254,3 +278,8 @@ uint32_t div_up(uint32_t a, uint32_t b);
 void read_status(Ctx &c);
 
 }  // namespace vtp
+
+// The opaque handle of the C ABI.
+struct vtp_ctx {
+    vtp::Ctx c;
+};

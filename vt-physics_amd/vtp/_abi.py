@@ -74,6 +74,7 @@ class VtpParams(C.Structure):
         ("cfl_number", C.c_float),
         ("dt_max", C.c_float),
         ("max_solver_iterations", C.c_uint32),
+        ("mass_weighted", C.c_int32),
     ]
 
 
