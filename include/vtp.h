@@ -179,6 +179,16 @@ vtp_status vtp_params_from_json(int32_t solver, const char *json_text,
 /* Default parameters = the solver's config template values. Host-only. */
 vtp_status vtp_params_default(int32_t solver, vtp_params *out);
 
+/* The solver's config template as JSON text (JsonHandler::load<Solver>
+ * ConfigTemplateJson, JSONHandler.cpp:65-67 and siblings; values of
+ * ConfigTemplates/<Solver>/<Solver>ConfigTemplate.json).  Static storage;
+ * NULL for an unknown solver.  Host-only. */
+const char *vtp_config_template(int32_t solver);
+/* The solver's object-component template (Solver::
+ * getSolverObjectComponentConfigTemplate, JSONHandler.cpp:69-71 and
+ * siblings).  Static storage; NULL for an unknown solver.  Host-only. */
+const char *vtp_object_config_template(int32_t solver);
+
 /* ---- lifecycle (Solver.hpp:27-62) ---------------------------------------- */
 /* SolverManager::createSolver + Solver::setConfig(json) */
 vtp_status vtp_create(vtp_ctx **out, int32_t solver, const char *json_text);

@@ -215,3 +215,22 @@ extern "C" vtp_status vtp_params_default(int32_t solver, vtp_params *out) {
     if (solver < 0 || solver > 3) return VTP_EINVAL;
     return vtp_params_from_json(solver, kTemplate[solver], out, nullptr, 0);
 }
+
+namespace {
+// Object-component templates
+// (ConfigTemplates/<Solver>/<Solver>ObjectComponentConfigTemplate.json).
+const char *kObjectTemplate[4] = {
+    R"({"solverType":0,"exportFlag":false,"velocityStart":[0,0,0],"colorStart":[0,1,0]})",
+    R"({"solverType":1,"exportFlag":false,"velocityStart":[0,0,0],"colorStart":[0,1,0]})",
+    R"({"solverType":2,"exportFlag":false,"velocityStart":[0,0,0],"phaseFraction":[0.1,0.1]})",
+    R"({"solverType":3,"exportFlag":false,"velocityStart":[0,0,0],"phaseFraction":[0.5,0.5]})",
+};
+}  // namespace
+
+extern "C" const char *vtp_config_template(int32_t solver) {
+    return (solver >= 0 && solver < 4) ? kTemplate[solver] : nullptr;
+}
+
+extern "C" const char *vtp_object_config_template(int32_t solver) {
+    return (solver >= 0 && solver < 4) ? kObjectTemplate[solver] : nullptr;
+}

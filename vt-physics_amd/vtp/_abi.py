@@ -7,6 +7,7 @@ If the library is missing the import fails loudly.
 from __future__ import annotations
 
 import ctypes as C
+import json
 import os
 from pathlib import Path
 
@@ -106,6 +107,8 @@ _U = C.POINTER(C.c_uint32)
 SIGNATURES = {
     "vtp_params_from_json": (C.c_int, [C.c_int32, C.c_char_p, C.POINTER(VtpParams), C.c_char_p, C.c_size_t]),
     "vtp_params_default": (C.c_int, [C.c_int32, C.POINTER(VtpParams)]),
+    "vtp_config_template": (C.c_char_p, [C.c_int32]),
+    "vtp_object_config_template": (C.c_char_p, [C.c_int32]),
     "vtp_create": (C.c_int, [C.POINTER(_P), C.c_int32, C.c_char_p]),
     "vtp_create_with_params": (C.c_int, [C.POINTER(_P), C.POINTER(VtpParams)]),
     "vtp_add_particles": (C.c_int, [_P, C.c_uint32, C.c_int32, _F, _F, _F, C.c_uint32]),
@@ -190,6 +193,24 @@ def params_default(solver: int | str) -> VtpParams:
     if st != VTP_OK:
         raise VtpError(st, "vtp_params_default failed")
     return p
+
+
+def config_template(solver: int | str) -> dict:
+    """The solver's config template (Solver::getSolverConfigTemplate)."""
+    sid = SOLVER_IDS[solver] if isinstance(solver, str) else solver
+    txt = load_library().vtp_config_template(sid)
+    if txt is None:
+        raise VtpError(VTP_EINVAL, f"unknown solver {solver!r}")
+    return json.loads(txt.decode())
+
+
+def object_config_template(solver: int | str) -> dict:
+    """Solver::getSolverObjectComponentConfigTemplate."""
+    sid = SOLVER_IDS[solver] if isinstance(solver, str) else solver
+    txt = load_library().vtp_object_config_template(sid)
+    if txt is None:
+        raise VtpError(VTP_EINVAL, f"unknown solver {solver!r}")
+    return json.loads(txt.decode())
 
 
 class Context:
