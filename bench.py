@@ -172,6 +172,8 @@ def cpu_baseline(wl, params, budget_s=20.0):
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle
     threads = os.cpu_count() or 1
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():   # the host share this job may use
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
     pyoracle.set_threads(threads)
     orc = pyoracle.Oracle.from_workload(params, wl)
     orc.initialize()
@@ -189,6 +191,21 @@ def cpu_baseline(wl, params, budget_s=20.0):
             "sample": f"FP64 oracle, {wl.name}: {steps} ticks after 1 untimed tick ({el:.1f} s)"}
 
 
+def calibrate_copy(torch, nbytes=1 << 30, reps=5):
+    """Plain device-to-device copies of a known size: the reference reading for
+    scaling the FETCH_SIZE / WRITE_SIZE counters of the kernels above."""
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(json.dumps({"copy_bytes_per_dispatch": nbytes, "read_kib": nbytes / 1024, "write_kib": nbytes / 1024,
+                      "dispatches": reps, "GBps": round(2 * nbytes * reps / el / 1e9, 1)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,6 +214,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--only-headline", action="store_true", help="skip the DFSPH / batched / adjoint / IMM-CT legs")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 1 GiB device copies (counter calibration for the traffic figure)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1 "
@@ -209,6 +228,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs an MI355X"
     torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        calibrate_copy(torch)
+        return
     wl = scenes.pbf_1m(seed=0)
     ctx, params = build_ctx(vtp, wl, graph=True)
     n, nf = wl.n, wl.n_fluid

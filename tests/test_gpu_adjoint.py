@@ -123,3 +123,50 @@ def test_torch_autograd_function():
     loss.backward()
     np.testing.assert_allclose(v0.grad.numpy(), ref_v0, rtol=1e-4, atol=1e-7)
     assert abs(float(k.grad) - ref_k) <= 1e-4 * abs(ref_k)
+
+
+def _cluster_ctx(k, K=1, xsph=0.0, seed=2):
+    """k^3 jittered fluid lattice at spacing 0.018 (no walls) with N(0, 0.1^2)
+    velocities, config-0 physics otherwise."""
+    s = 0.018
+    g = np.stack(np.meshgrid(*(np.arange(k) * s,) * 3, indexing="ij"), -1).reshape(-1, 3).astype(np.float32)
+    pos = scenes.jitter(g + 0.3, s, seed=seed)
+    vel = np.random.default_rng(seed).normal(0, 0.1, pos.shape).astype(np.float32)
+    cfg = scenes.pbf_config(lb=(-0.5, -0.5, -0.5), size=(2, 2, 2), iterations=K, xsph=xsph)
+    import json
+    p = vtp.params_from_json("PBF", json.dumps(cfg))
+    ctx = vtp.Context(p)
+    ctx.add_particles(pos, vtp.MAT_FLUID, vel)
+    ctx.initialize()
+    n = pos.shape[0]
+    o = pyoracle.Oracle(p, pos, vel, None, None, (np.array([0], np.uint32), np.array([n], np.uint32),
+                                                   np.array([vtp.MAT_FLUID], np.int32)))
+    o.initialize()
+    return ctx, o, n
+
+
+def test_gpu_adjoint_fully_clamped_cluster():
+    """Eight particles never reach rest density: every constraint is clamped
+    (C = 0, lambda = 0), so the constraint Jacobian vanishes and a velocity
+    loss pulls back to the initial positions with zero gradient."""
+    ctx, o, n = _cluster_ctx(2)
+    cv = np.random.default_rng(1).normal(size=(n, 3))
+    ctx.rollout_fwd(1, 1)
+    gv0, gx0, _ = ctx.rollout_bwd(None, cv)
+    assert np.all(gx0 == 0.0)
+    np.testing.assert_allclose(gv0, cv, rtol=1e-6)
+
+
+@pytest.mark.parametrize("k,K,xsph,steps", [(5, 1, 0.0, 1), (8, 1, 0.0, 1), (8, 4, 0.01, 1), (8, 4, 0.01, 3)])
+def test_gpu_adjoint_cluster_matches_fp64(k, K, xsph, steps):
+    ctx, o, n = _cluster_ctx(k, K, xsph)
+    rng = np.random.default_rng(3)
+    cx, cv = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+    ctx.rollout_fwd(steps, 2)
+    gv0, gx0, gk = ctx.rollout_bwd(cx, cv)
+    o.rollout_fwd(steps)
+    rv0, rx0, rk = o.rollout_bwd(cx, cv)
+    assert np.linalg.norm(gx0 - rx0) <= 1e-3 * np.linalg.norm(rx0)
+    assert np.linalg.norm(gv0 - rv0) <= 1e-3 * np.linalg.norm(rv0)
+    if xsph:
+        assert abs(gk - rk) <= 1e-3 * abs(rk)

@@ -76,6 +76,22 @@ def test_pbf_one_step_cfg0(graph):
     _check_fields(ctx, orc, TOL_1)
 
 
+def test_pbf_clamped_particles_have_zero_error():
+    """Where the oracle clamps the density (rho <= rho0, C = 0 exactly), the
+    device C and lambda are exactly 0 too: the reference divides rho by rho0
+    (PBFCudaImpl.cu:92), so a clamped particle gets 1000/1000 - 1 = 0."""
+    wl = scenes.cfg0_pbf(n_side=(12, 12, 12))
+    ctx, orc = _pair(wl, graph=1)
+    ctx.step(1)
+    orc.step(1)
+    nf = wl.n_fluid
+    Co = orc.read(7)[:nf]
+    clamped = Co == 0.0
+    assert 0 < clamped.sum() < nf
+    assert np.all(ctx.read("error")[:nf][clamped] == 0.0)
+    assert np.all(ctx.read("lambda")[:nf][clamped] == 0.0)
+
+
 def test_pbf_twenty_steps():
     wl = scenes.cfg0_pbf(n_side=(20, 20, 20))
     ctx, orc = _pair(wl, graph=1)

@@ -84,7 +84,7 @@ __global__ __launch_bounds__(256) void k_adj_xsph(const float4 *__restrict__ X, 
                                                   const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ cnt,
                                                   uint32_t n, uint32_t stride, PbfK q, float4 *__restrict__ xb,
                                                   float4 *__restrict__ vb, double *kbar) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t p = slot_index();
     double kp = 0.0;
     if (p < n && mat[p] == VTP_MAT_FLUID) {
         const float4 xp = X[p], vp = V[p], up = U[p];
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void k_adj_lambar(const float4 *__restrict__ X
                                                     const int32_t *__restrict__ mat, const uint32_t *__restrict__ nbr,
                                                     const uint32_t *__restrict__ cnt, uint32_t n, uint32_t stride,
                                                     PbfK q, float2 *__restrict__ rs) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = slot_index();
     if (i >= n) return;
     if (mat[i] != VTP_MAT_FLUID) {
         rs[i] = make_float2(0, 0);
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) void k_adj_xbar(const float4 *__restrict__ X, 
                                                   const float4 *__restrict__ sg, const int32_t *__restrict__ mat,
                                                   const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ cnt,
                                                   uint32_t n, uint32_t stride, PbfK q, float4 *__restrict__ xb) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t p = slot_index();
     if (p >= n || mat[p] != VTP_MAT_FLUID) return;
     const float4 xp = X[p], wp = Wv[p], Gp = sg[p];
     const float2 rp = rs[p];

@@ -137,7 +137,7 @@ __global__ __launch_bounds__(256) void k_df_precompute(const float4 *__restrict_
                                                        const uint32_t *__restrict__ cnt, uint32_t n,
                                                        uint32_t stride, DfK q, float *__restrict__ cr,
                                                        float *__restrict__ alpha) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = slot_index();
     if (i >= n) return;
     const float4 vi = v[i];
     vadv[i] = vi;
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(256) void k_df_delta_kappa(const float4 *__restrict
                                                         const float *__restrict__ alpha, float *__restrict__ delta,
                                                         float *__restrict__ kappa, Loop lp) {
     if (lp.status[kStDone]) return;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = slot_index();
     const float dt = lp.sc->dt;
     bool active = false;
     double contrib = 0.0;
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(256) void k_df_update_vadv(const float4 *__restrict
                                                         uint32_t stride, Cubic k, const float *__restrict__ kappa,
                                                         Loop lp) {
     if (lp.status[kStDone]) return;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = slot_index();
     if (i >= n || mat[i] != VTP_MAT_FLUID) return;
     const float dt = lp.sc->dt;
     const float4 xi = x[i];
@@ -350,7 +350,7 @@ __global__ __launch_bounds__(256) void k_df_vis_normal(const float4 *__restrict_
                                                        const uint32_t *__restrict__ cnt, uint32_t n,
                                                        uint32_t stride, DfK q, float4 *__restrict__ acc,
                                                        float4 *__restrict__ normal) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = slot_index();
     if (i >= n) return;
     const int32_t mi = mat[i];
     if (mi != VTP_MAT_FLUID) {
@@ -402,7 +402,7 @@ __global__ __launch_bounds__(256) void k_df_tension_apply(const float4 *__restri
                                                           const uint32_t *__restrict__ cnt, uint32_t n,
                                                           uint32_t stride, DfK q, float4 *__restrict__ acc,
                                                           const SolverScalars *sc) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = slot_index();
     if (i >= n) return;
     const int32_t mi = mat[i];
     if (mi != VTP_MAT_FLUID) return;

@@ -190,7 +190,7 @@ __global__ __launch_bounds__(256) void k_imm_precompute(const float4 *__restrict
                                                         uint32_t stride, ImmK q, const float *__restrict__ fpoly,
                                                         float *__restrict__ cr, float *__restrict__ alpha,
                                                         float *__restrict__ cd) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = slot_index();
     if (i >= n) return;
     const float4 vi = v[i];
     vadv[i] = vi;
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(256) void k_imm_vis_normal(const float4 *__restrict
                                                         const uint32_t *__restrict__ cnt, uint32_t n,
                                                         uint32_t stride, ImmK q, Ph ph,
                                                         float4 *__restrict__ normal) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = slot_index();
     if (i >= n) return;
     const int32_t mi = mat[i];
     if (mi != VTP_MAT_FLUID) return;
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(256) void k_imm_tension_phase_vel(const float4 *__r
                                                                const uint32_t *__restrict__ cnt, uint32_t n,
                                                                uint32_t stride, ImmK q, Ph ph,
                                                                const SolverScalars *sc) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = slot_index();
     if (i >= n) return;
     const int32_t mi = mat[i];
     if (mi != VTP_MAT_FLUID) return;
@@ -399,7 +399,7 @@ __global__ __launch_bounds__(256) void k_immct_conformation(const float4 *__rest
                                                             float *__restrict__ Q, float *__restrict__ Lout,
                                                             float *__restrict__ stress, float *__restrict__ vis,
                                                             const SolverScalars *sc) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = slot_index();
     if (i >= n) return;
     if (mat[i] != VTP_MAT_FLUID) {
         vis[i] = q.vis_base;
@@ -477,7 +477,7 @@ __global__ __launch_bounds__(256) void k_immct_stress_acc(const float4 *__restri
                                                           const uint32_t *__restrict__ nbr,
                                                           const uint32_t *__restrict__ cnt, uint32_t n,
                                                           uint32_t stride, ImmK q, Ph ph, const SolverScalars *sc) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = slot_index();
     if (i >= n) return;
     if (mat[i] != VTP_MAT_FLUID) return;
     const float dt = sc->dt;
@@ -538,7 +538,7 @@ __global__ __launch_bounds__(256) void k_imm_fluxes(const float4 *__restrict__ x
                                                     const uint32_t *__restrict__ cnt, uint32_t n, uint32_t stride,
                                                     ImmK q, Ph ph, const uint32_t *status, const SolverScalars *sc) {
     if (status[kStDone]) return;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = slot_index();
     if (i >= n) return;
     const int32_t mi = mat[i];
     if (mi != VTP_MAT_FLUID) return;

@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void k_pbf_lambda(const float4 *__restrict__ x
                                                     uint32_t stride, PbfK q, float *__restrict__ lam,
                                                     float *__restrict__ dens, float *__restrict__ errc,
                                                     float4 *__restrict__ sg) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = slot_index();
     if (i >= n) return;
     if (mat[i] != VTP_MAT_FLUID) {
         // slots are permuted every tick, so non-fluid slots are written too
@@ -104,7 +104,10 @@ __global__ __launch_bounds__(256) void k_pbf_lambda(const float4 *__restrict__ x
             s2 += ax * ax + ay * ay + az * az;
         });
     rho = fmaxf(rho, q.rho0);
-    float C = rho * q.inv_rho0 - 1.0f;
+    // rho/rho0 - 1 with a rounded division, as the reference (PBFCudaImpl.cu:92):
+    // a clamped particle gets exactly C = 0 (a fused rho*(1/rho0) - 1 would
+    // leave 1000*0.001f - 1 = 4.7e-8, and the adjoint tests C > 0 for the clamp).
+    float C = __fdiv_rn(rho, q.rho0) - 1.0f;
     float ir2 = q.inv_rho0 * q.inv_rho0;
     float S = (s2 + gsx * gsx + gsy * gsy + gsz * gsz) * ir2;
     lam[i] = -C / (S + 1e-6f);
@@ -121,7 +124,7 @@ __global__ __launch_bounds__(256) void k_pbf_dx(const float4 *__restrict__ x, fl
                                                 const uint32_t *__restrict__ nbr,
                                                 const uint32_t *__restrict__ cnt, uint32_t n, uint32_t stride,
                                                 PbfK q, const float *__restrict__ lam) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = slot_index();
     if (i >= n) return;
     if (mat[i] != VTP_MAT_FLUID) return;
     const float4 xi = x[i];
@@ -164,7 +167,7 @@ __global__ __launch_bounds__(256) void k_pbf_xsph(const float4 *__restrict__ x, 
                                                   const uint32_t *__restrict__ nbr,
                                                   const uint32_t *__restrict__ cnt, uint32_t n,
                                                   uint32_t stride, PbfK q) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = slot_index();
     if (i >= n) return;
     const float4 xi = x[i];
     const float4 vi = v[i];

@@ -38,7 +38,7 @@ void autotune(Ctx &c) {
     const bool graph_was = c.opt_graph;
     c.opt_graph = false;
     std::map<std::string, std::pair<double, uint32_t>> best;
-    const uint32_t cand[] = {64, 128, 256, 512};
+    const uint32_t cand[] = {64, 128, 256};   // the passes are built for at most 256 threads
     for (uint32_t b : cand) {
         for (const auto &s : stages) c.block_of[s] = b;
         c.timer.reset();

@@ -24,6 +24,11 @@ constexpr int kRsThreads = 256;
 constexpr int kRsRounds = 16;                   // keys per lane per tile
 constexpr int kRsWaveChunk = 64 * kRsRounds;    // 1024 keys per wavefront
 constexpr int kRsTile = kRsThreads * kRsRounds; // 4096 keys per workgroup
+// Neighbour build: 128 threads, each with a private LDS column of `cap`
+// words (30 KB at cap 59, five workgroups per CU); larger caps write the
+// lists straight to global memory.
+constexpr uint32_t kNbrBuildThreads = 128;
+constexpr size_t kNbrBuildMaxLds = 64 * 1024;
 
 __global__ void k_fill_u32(uint32_t *a, uint32_t n, uint32_t v) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -208,7 +213,7 @@ __global__ void k_cell_fixup(const uint32_t *__restrict__ key, const uint32_t *_
 // slot order that passes 0 < |x_i - x_j| <= h, capped at `cap` entries.
 __device__ uint32_t reference_scan(const float4 *__restrict__ x, const uint32_t *__restrict__ start,
                                    const uint32_t *__restrict__ end, const Grid &g, uint32_t i, float3 p, int cx,
-                                   int cy, int cz, uint32_t sbase, uint32_t stride, uint32_t cap,
+                                   int cy, int cz, uint32_t sbase, uint32_t stride, uint32_t row, uint32_t cap,
                                    uint32_t *__restrict__ nbr) {
     const uint32_t gxy = (uint32_t)(g.gx * g.gy);
     uint32_t c = 0;
@@ -229,7 +234,7 @@ __device__ uint32_t reference_scan(const float4 *__restrict__ x, const uint32_t 
                     if (j == i) continue;
                     float d2 = dist2_exact(p, xyz(x[j]));
                     if (is_neighbor(d2, g.nbr_d2_max) && c < cap) {
-                        nbr[(size_t)c * stride + i] = j;
+                        nbr[(size_t)c * stride + row] = j;
                         ++c;
                     }
                 }
@@ -247,66 +252,85 @@ __device__ uint32_t reference_scan(const float4 *__restrict__ x, const uint32_t 
 // with the reference traversal so the truncated set is the reference's.
 // (The order inside a list only matters for truncation; the reference-layout
 // export re-sorts every list into the reference order.)
-__global__ __launch_bounds__(256) void k_build_neighbors(const float4 *__restrict__ x,
-                                                         const uint32_t *__restrict__ key,
-                                                         const uint32_t *__restrict__ start,
-                                                         const uint32_t *__restrict__ end, Grid g, uint32_t n,
-                                                         uint32_t stride, uint32_t cap,
-                                                         uint32_t *__restrict__ nbr, uint32_t *__restrict__ cnt) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t k = key[i];
-    if (k >= g.dead_key) {
-        cnt[i] = kDeadBit;
-        return;
-    }
-    const uint32_t gxy = (uint32_t)(g.gx * g.gy);
-    uint32_t scene = k / g.cells;
-    uint32_t local = k - scene * g.cells;
-    int cz = (int)(local / gxy);
-    uint32_t rem = local - (uint32_t)cz * gxy;
-    int cy = (int)(rem / (uint32_t)g.gx);
-    int cx = (int)(rem - (uint32_t)cy * (uint32_t)g.gx);
-    const uint32_t sbase = scene * g.cells;
-    const float3 p = xyz(x[i]);
-    const int X0 = max(cx - 1, 0), X1 = min(cx + 1, g.gx - 1);
+//
+// kStaged: each thread assembles its list in a private LDS column
+// (s_list[c * blockDim + tid]) and the wave then writes the ELL columns out
+// one at a time, so every store of the wave covers 64 consecutive slots of
+// one column.  Written straight to HBM, the lanes' appends land in `cap`
+// different columns at once and the L2 evicts most lines half-written.
+template <bool kStaged>
+__global__ __launch_bounds__(kNbrBuildThreads) void k_build_neighbors(const float4 *__restrict__ x,
+                                                                      const uint32_t *__restrict__ key,
+                                                                      const uint32_t *__restrict__ start,
+                                                                      const uint32_t *__restrict__ end, Grid g,
+                                                                      uint32_t n, uint32_t stride, uint32_t cap,
+                                                                      uint32_t *__restrict__ nbr,
+                                                                      uint32_t *__restrict__ cnt) {
+    extern __shared__ uint32_t s_list[];
+    const uint32_t i = slot_index();
+    uint32_t *const dst = kStaged ? s_list : nbr;
+    const uint32_t dstride = kStaged ? blockDim.x : stride;
+    const uint32_t drow = kStaged ? threadIdx.x : i;
     uint32_t c = 0;
-    auto test = [&](uint32_t j, const float4 &xj) {
-        float d2 = dist2_exact(p, xyz(xj));
-        if (j != i && is_neighbor(d2, g.nbr_d2_max)) {
-            if (c < cap) nbr[(size_t)c * stride + i] = j;
-            ++c;
-        }
-    };
-    for (int oy = -1; oy <= 1; ++oy) {
-        int Y = cy + oy;
-        if (Y < 0 || Y >= g.gy) continue;
-        for (int oz = -1; oz <= 1; ++oz) {
-            int Z = cz + oz;
-            if (Z < 0 || Z >= g.gz) continue;
-            const uint32_t row = sbase + (uint32_t)Z * gxy + (uint32_t)Y * (uint32_t)g.gx;
-            uint32_t s = kEmpty, e = 0;
-            for (int X = X0; X <= X1; ++X) {
-                uint32_t a = start[row + X];
-                if (a != kEmpty) {
-                    if (s == kEmpty) s = a;
-                    e = end[row + X];
+    if (i < n) {
+        const uint32_t k = key[i];
+        if (k >= g.dead_key) {
+            cnt[i] = kDeadBit;
+        } else {
+            const uint32_t gxy = (uint32_t)(g.gx * g.gy);
+            const uint32_t scene = k / g.cells;
+            const uint32_t local = k - scene * g.cells;
+            const int cz = (int)(local / gxy);
+            const uint32_t rem = local - (uint32_t)cz * gxy;
+            const int cy = (int)(rem / (uint32_t)g.gx);
+            const int cx = (int)(rem - (uint32_t)cy * (uint32_t)g.gx);
+            const uint32_t sbase = scene * g.cells;
+            const float3 p = xyz(x[i]);
+            const int X0 = max(cx - 1, 0), X1 = min(cx + 1, g.gx - 1);
+            auto test = [&](uint32_t j, const float4 &xj) {
+                float d2 = dist2_exact(p, xyz(xj));
+                if (j != i && is_neighbor(d2, g.nbr_d2_max)) {
+                    if (c < cap) dst[(size_t)c * dstride + drow] = j;
+                    ++c;
+                }
+            };
+            for (int oy = -1; oy <= 1; ++oy) {
+                const int Y = cy + oy;
+                if (Y < 0 || Y >= g.gy) continue;
+                for (int oz = -1; oz <= 1; ++oz) {
+                    const int Z = cz + oz;
+                    if (Z < 0 || Z >= g.gz) continue;
+                    const uint32_t row = sbase + (uint32_t)Z * gxy + (uint32_t)Y * (uint32_t)g.gx;
+                    uint32_t s = kEmpty, e = 0;
+                    for (int X = X0; X <= X1; ++X) {
+                        uint32_t a = start[row + X];
+                        if (a != kEmpty) {
+                            if (s == kEmpty) s = a;
+                            e = end[row + X];
+                        }
+                    }
+                    if (s == kEmpty) continue;
+                    uint32_t j = s;
+                    for (; j + 4 <= e + 1; j += 4) {
+                        float4 a0 = x[j], a1 = x[j + 1], a2 = x[j + 2], a3 = x[j + 3];
+                        test(j, a0);
+                        test(j + 1, a1);
+                        test(j + 2, a2);
+                        test(j + 3, a3);
+                    }
+                    for (; j <= e; ++j) test(j, x[j]);
                 }
             }
-            if (s == kEmpty) continue;
-            uint32_t j = s;
-            for (; j + 4 <= e + 1; j += 4) {
-                float4 a0 = x[j], a1 = x[j + 1], a2 = x[j + 2], a3 = x[j + 3];
-                test(j, a0);
-                test(j + 1, a1);
-                test(j + 2, a2);
-                test(j + 3, a3);
-            }
-            for (; j <= e; ++j) test(j, x[j]);
+            if (c > cap) c = reference_scan(x, start, end, g, i, p, cx, cy, cz, sbase, dstride, drow, cap, dst);
+            cnt[i] = c;
         }
     }
-    if (c > cap) c = reference_scan(x, start, end, g, i, p, cx, cy, cz, sbase, stride, cap, nbr);
-    cnt[i] = c;
+    if (kStaged) {
+        // every lane reads back only its own LDS column: no barrier needed
+        const uint32_t cmax = wave_max_u32(c);
+        for (uint32_t col = 0; col < cmax; ++col)
+            if (col < c) nbr[(size_t)col * stride + i] = s_list[col * blockDim.x + threadIdx.x];
+    }
 }
 
 // Reference-layout export (UniformGridNeighborSearch.hpp:28-63): attach ids,
@@ -419,8 +443,13 @@ void Ugns::sort_and_rank(const uint32_t *orig_old, hipStream_t s, StageTimer &tm
 
 void Ugns::build_neighbors(const float4 *x, hipStream_t s, StageTimer &tm) {
     tm.begin(s, "ugns_neighbors");
-    hipLaunchKernelGGL(k_build_neighbors, dim3(blocks_for(n, 256)), dim3(256), 0, s, x, key[sorted].p, start.p,
-                       end.p, g, n, stride, cap, nbr.p, cnt.p);
+    const size_t lds = (size_t)cap * kNbrBuildThreads * sizeof(uint32_t);
+    if (lds <= kNbrBuildMaxLds)
+        hipLaunchKernelGGL(k_build_neighbors<true>, dim3(blocks_for(n, kNbrBuildThreads)), dim3(kNbrBuildThreads),
+                           lds, s, x, key[sorted].p, start.p, end.p, g, n, stride, cap, nbr.p, cnt.p);
+    else
+        hipLaunchKernelGGL(k_build_neighbors<false>, dim3(blocks_for(n, kNbrBuildThreads)), dim3(kNbrBuildThreads),
+                           0, s, x, key[sorted].p, start.p, end.p, g, n, stride, cap, nbr.p, cnt.p);
     tm.end(s);
 }
 

@@ -89,9 +89,10 @@ __host__ inline Cubic make_cubic(float h) {
 
 // W(r) from r^2.  q <= 0.5: sigma(6q^3 - 6q^2 + 1); q <= 1: 2 sigma (1-q)^3.
 // Both branches are evaluated and selected (no divergent branch in the
-// neighbour loops).
+// neighbour loops).  r = r2 * rsqrt(r2) (one transcendental; a correctly
+// rounded sqrt costs a dozen instructions and changes W by < 1e-7 relative).
 __device__ __forceinline__ float cubic_w(float r2, const Cubic &k) {
-    float r = __fsqrt_rn(r2);
+    float r = r2 > 0.0f ? r2 * __frsqrt_rn(r2) : 0.0f;
     float q = r * k.inv_h;
     float t = 1.0f - q;
     float wi = fmaf(6.0f * q * q, q - 1.0f, 1.0f);
@@ -140,6 +141,21 @@ __device__ __forceinline__ float surface_tension_c(float r, float h, float c_sig
     return 0.0f;
 }
 
+// ---- XCD-aware slot mapping ---------------------------------------------------
+// The dispatcher deals workgroups round-robin over the 8 XCDs of the die
+// (workgroup b runs on XCD b % 8), and each XCD has its own L2.  Slots are in
+// cell order, so giving every XCD one contiguous eighth of the workgroups
+// keeps a particle's neighbours (and the neighbours of its neighbours) in the
+// same L2 instead of spreading every neighbourhood over all eight.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block() {
+    const uint32_t b = blockIdx.x, nb = gridDim.x;
+    const uint32_t x = b % kXcds, k = b / kXcds;   // XCD, rank inside that XCD
+    const uint32_t q = nb / kXcds, r = nb % kXcds; // the first r XCDs get q + 1 blocks
+    return x * q + min(x, r) + k;
+}
+__device__ __forceinline__ uint32_t slot_index() { return xcd_block() * blockDim.x + threadIdx.x; }
+
 // ---- neighbour iteration ---------------------------------------------------
 // Walks the ELL list of slot i in batches of four: the four indices are
 // loaded first, then the four gathers are issued, then the four bodies run,
@@ -176,6 +192,11 @@ __device__ __forceinline__ double wave_sum(double v) {
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off, 64));
     return v;
 }
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
