@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void k_adj_xsph(const float4 *__restrict__ X, 
         float ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0, sw = 0, kk = 0;
         struct D { float4 x, v, u; };
         for_each_neighbor(
-            nbr, stride, p, nn, [&](uint32_t j) { return D{X[j], V[j], U[j]}; },
+            nbr, stride, p, nn, [&](uint32_t j) { return D{ldg(X, j), ldg(V, j), ldg(U, j)}; },
             [&](const D &e) {
                 float dx = xp.x - e.x.x, dy = xp.y - e.x.y, dz = xp.z - e.x.z;
                 float W, f;
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(256) void k_adj_lambar(const float4 *__restrict__ X
     float lb = 0.0f;
     struct D { float4 x, w; };
     for_each_neighbor(
-        nbr, stride, i, nn, [&](uint32_t j) { return D{X[j], Wv[j]}; },
+        nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(X, j), ldg(Wv, j)}; },
         [&](const D &e) {
             float dx = xi.x - e.x.x, dy = xi.y - e.x.y, dz = xi.z - e.x.z;
             float f = cubic_grad_factor(dx * dx + dy * dy + dz * dz, q.k);
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void k_adj_xbar(const float4 *__restrict__ X, 
     float ax = 0, ay = 0, az = 0;
     struct D { float4 x, w, G; float2 r; float l; };
     for_each_neighbor(
-        nbr, stride, p, nn, [&](uint32_t j) { return D{X[j], Wv[j], sg[j], rs[j], lam[j]}; },
+        nbr, stride, p, nn, [&](uint32_t j) { return D{ldg(X, j), ldg(Wv, j), ldg(sg, j), ldg(rs, j), ldg(lam, j)}; },
         [&](const D &e) {
             float rx = xp.x - e.x.x, ry = xp.y - e.x.y, rz = xp.z - e.x.z;
             float f, d;
@@ -230,7 +230,7 @@ __global__ void k_dl_rho(const float4 *__restrict__ X, const int32_t *__restrict
             const uint32_t nn = w & ~kDeadBit;
             float rho = (w & kDeadBit) ? 0.0f : xi.w * q.k.w0;
             for_each_neighbor(
-                nbr, stride, i, nn, [&](uint32_t j) { return X[j]; },
+                nbr, stride, i, nn, [&](uint32_t j) { return ldg(X, j); },
                 [&](const float4 &xj) {
                     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
                     rho = fmaf(xj.w, cubic_w(dx * dx + dy * dy + dz * dz, q.k), rho);
@@ -256,7 +256,7 @@ __global__ void k_dl_grad(const float4 *__restrict__ X, const float *__restrict_
         const uint32_t nn = cnt[p] & ~kDeadBit;
         struct D { float4 x; float r; };
         for_each_neighbor(
-            nbr, stride, p, nn, [&](uint32_t j) { return D{X[j], rb[j]}; },
+            nbr, stride, p, nn, [&](uint32_t j) { return D{ldg(X, j), ldg(rb, j)}; },
             [&](const D &e) {
                 float dx = xp.x - e.x.x, dy = xp.y - e.x.y, dz = xp.z - e.x.z;
                 float f = cubic_grad_factor(dx * dx + dy * dy + dz * dz, q.k) * (rp * e.x.w + e.r * xp.w);

@@ -277,7 +277,9 @@ extern "C" vtp_status vtp_add_particles(vtp_ctx *ctx, uint32_t scene, int32_t ma
     if (mat != VTP_MAT_FLUID && mat != VTP_MAT_BOUNDARY)
         return ctx_fail(c, VTP_EINVAL, "vtp_add_particles: unsupported material");  // PBFSolver.cpp:175-178
     if (c.objects.size() >= VTP_MAX_OBJECTS) return ctx_fail(c, VTP_EINVAL, "vtp_add_particles: too many objects");
-    if ((uint64_t)c.n + n >= 0x7FFFFFFFull) return ctx_fail(c, VTP_EINVAL, "vtp_add_particles: too many particles");
+    // neighbour gathers address a particle array with a 32-bit byte offset (16 B elements)
+    if ((uint64_t)c.n + n > (1ull << 28))
+        return ctx_fail(c, VTP_EINVAL, "vtp_add_particles: at most 2^28 particles per context");
     return guarded(c, [&]() {
         Object o{c.n, c.n + n, scene, mat};
         c.h_pos.insert(c.h_pos.end(), xyz, xyz + 3 * (size_t)n);

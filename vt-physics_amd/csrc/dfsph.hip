@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256) void k_df_precompute(const float4 *__restrict_
     float a1x = 0, a1y = 0, a1z = 0, a2 = 1e-6f;
     struct D { float4 x; float V; int32_t m; };
     for_each_neighbor(
-        nbr, stride, i, nn, [&](uint32_t j) { return D{x[j], v[j].w, mat[j]}; },
+        nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg_w(v, j), ldg(mat, j)}; },
         [&](const D &d) {
             float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
             float W, f;
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(256) void k_df_delta_kappa(const float4 *__restrict
             struct D { float4 x, a; };
             for_each_neighbor(
                 nbr, stride, i, nn,
-                [&](uint32_t j) { return D{x[j], make_float4(vadv[j].x, vadv[j].y, vadv[j].z, v[j].w)}; },
+                [&](uint32_t j) { return D{ldg(x, j), make_float4(ldg(vadv, j).x, ldg(vadv, j).y, ldg(vadv, j).z, ldg_w(v, j))}; },
                 [&](const D &e) {
                     float dx = xi.x - e.x.x, dy = xi.y - e.x.y, dz = xi.z - e.x.z;
                     float f = cubic_grad_factor(dx * dx + dy * dy + dz * dz, k);
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(256) void k_df_update_vadv(const float4 *__restrict
     float ax = 0, ay = 0, az = 0;
     struct D { float4 x; float k; };
     for_each_neighbor(
-        nbr, stride, i, nn, [&](uint32_t j) { return D{make_float4(x[j].x, x[j].y, x[j].z, v[j].w), kappa[j]}; },
+        nbr, stride, i, nn, [&](uint32_t j) { return D{make_float4(ldg(x, j).x, ldg(x, j).y, ldg(x, j).z, ldg_w(v, j)), ldg(kappa, j)}; },
         [&](const D &e) {
             float dx = xi.x - e.x.x, dy = xi.y - e.x.y, dz = xi.z - e.x.z;
             float s = cubic_grad_factor(dx * dx + dy * dy + dz * dz, k) * e.x.w * (ki + e.k);
@@ -367,8 +367,8 @@ __global__ __launch_bounds__(256) void k_df_vis_normal(const float4 *__restrict_
     for_each_neighbor(
         nbr, stride, i, nn,
         [&](uint32_t j) {
-            float4 xj = x[j];
-            return D{xj, v[j], xj.w / rest[j], vis[j], mat[j]};
+            float4 xj = ldg(x, j);
+            return D{xj, ldg(v, j), xj.w / ldg(rest, j), ldg(vis, j), ldg(mat, j)};
         },
         [&](const D &d) {
             if (d.m != mi) return;
@@ -415,7 +415,7 @@ __global__ __launch_bounds__(256) void k_df_tension_apply(const float4 *__restri
         float sx = 0, sy = 0, sz = 0;
         struct D { float4 x, n; float r; int32_t m; };
         for_each_neighbor(
-            nbr, stride, i, nn, [&](uint32_t j) { return D{x[j], normal[j], rest[j], mat[j]}; },
+            nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg(normal, j), ldg(rest, j), ldg(mat, j)}; },
             [&](const D &d) {
                 if (d.m != mi) return;
                 float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;

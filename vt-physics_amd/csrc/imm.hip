@@ -208,7 +208,7 @@ __global__ __launch_bounds__(256) void k_imm_precompute(const float4 *__restrict
     float a1x = 0, a1y = 0, a1z = 0, a2 = 1e-6f, b = 0.0f;
     struct D { float4 x; float V, fp; int32_t m; };
     for_each_neighbor(
-        nbr, stride, i, nn, [&](uint32_t j) { return D{x[j], v[j].w, q.ct ? fpoly[j] : 0.0f, mat[j]}; },
+        nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg_w(v, j), q.ct ? ldg(fpoly, j) : 0.0f, ldg(mat, j)}; },
         [&](const D &d) {
             float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
             float W, f;
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(256) void k_imm_vis_normal(const float4 *__restrict
     float nx = 0, ny = 0, nz = 0;
     struct D { float4 x, v; float r; int32_t m; };
     for_each_neighbor(
-        nbr, stride, i, nn, [&](uint32_t j) { return D{x[j], v[j], rest[j], mat[j]}; },
+        nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg(v, j), ldg(rest, j), ldg(mat, j)}; },
         [&](const D &d) {
             if (d.m != mi) return;
             float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
@@ -353,7 +353,7 @@ __global__ __launch_bounds__(256) void k_imm_tension_phase_vel(const float4 *__r
         const uint32_t nn = cnt[i] & ~kDeadBit;
         struct D { float4 x, n; float r; int32_t m; };
         for_each_neighbor(
-            nbr, stride, i, nn, [&](uint32_t j) { return D{x[j], normal[j], rest[j], mat[j]}; },
+            nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg(normal, j), ldg(rest, j), ldg(mat, j)}; },
             [&](const D &d) {
                 if (d.m != mi) return;
                 float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
@@ -412,7 +412,7 @@ __global__ __launch_bounds__(256) void k_immct_conformation(const float4 *__rest
     float L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     struct D { float4 x, v; float r; int32_t m; };
     for_each_neighbor(
-        nbr, stride, i, nn, [&](uint32_t j) { return D{x[j], v[j], rest[j], mat[j]}; },
+        nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg(v, j), ldg(rest, j), ldg(mat, j)}; },
         [&](const D &d) {
             if (d.m != VTP_MAT_FLUID) return;
             float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;

@@ -72,24 +72,25 @@ __global__ __launch_bounds__(256) void k_pbf_lambda(const float4 *__restrict__ x
                                                     const uint32_t *__restrict__ cnt, uint32_t n,
                                                     uint32_t stride, PbfK q, float *__restrict__ lam,
                                                     float *__restrict__ dens, float *__restrict__ errc,
-                                                    float4 *__restrict__ sg) {
+                                                    float4 *__restrict__ sg, float4 *__restrict__ xl) {
     uint32_t i = slot_index();
     if (i >= n) return;
+    const float4 xi = x[i];
     if (mat[i] != VTP_MAT_FLUID) {
         // slots are permuted every tick, so non-fluid slots are written too
         lam[i] = 0.0f;
         dens[i] = 0.0f;
         errc[i] = 0.0f;
+        xl[i] = make_float4(xi.x, xi.y, xi.z, 0.0f);
         if (sg) sg[i] = make_float4(0, 0, 0, 0);
         return;
     }
-    const float4 xi = x[i];
     const uint32_t w = cnt[i];
     const uint32_t nn = w & ~kDeadBit;
     float rho = (w & kDeadBit) ? 0.0f : xi.w * q.k.w0;
     float gsx = 0, gsy = 0, gsz = 0, s2 = 0;
     for_each_neighbor(
-        nbr, stride, i, nn, [&](uint32_t j) { return x[j]; },
+        nbr, stride, i, nn, [&](uint32_t j) { return ldg(x, j); },
         [&](const float4 &xj) {
             float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
             float r2 = dx * dx + dy * dy + dz * dz;
@@ -110,7 +111,9 @@ __global__ __launch_bounds__(256) void k_pbf_lambda(const float4 *__restrict__ x
     float C = __fdiv_rn(rho, q.rho0) - 1.0f;
     float ir2 = q.inv_rho0 * q.inv_rho0;
     float S = (s2 + gsx * gsx + gsy * gsy + gsz * gsz) * ir2;
-    lam[i] = -C / (S + 1e-6f);
+    const float li = -C / (S + 1e-6f);
+    lam[i] = li;
+    xl[i] = make_float4(xi.x, xi.y, xi.z, li);   // the delta-x pass gathers (x_j, lambda_j) in one load
     dens[i] = rho;
     errc[i] = C;
     if (sg) sg[i] = make_float4(gsx, gsy, gsz, S);   // taped for the adjoint
@@ -119,23 +122,29 @@ __global__ __launch_bounds__(256) void k_pbf_lambda(const float4 *__restrict__ x
 // compute_dx_cuda + apply_dx_cuda (PBFCudaImpl.cu:124-170):
 //   dx_i = (1/rho0) sum_j (lambda_i + lambda_j) gradW_ij,
 //   x_i <- x_i + dx_i,  v_i <- v_i + dx_i / dt.
-__global__ __launch_bounds__(256) void k_pbf_dx(const float4 *__restrict__ x, float4 *__restrict__ xo,
-                                                float4 *__restrict__ v, const int32_t *__restrict__ mat,
-                                                const uint32_t *__restrict__ nbr,
+// Neighbours come from xl = (x, lambda) written by the lambda pass: one
+// 16-byte gather per neighbour.  MW (extension) weights the pair by m_j, read
+// from the position buffer.
+template <bool MW>
+__global__ __launch_bounds__(256) void k_pbf_dx(const float4 *__restrict__ x, const float4 *__restrict__ xl,
+                                                float4 *__restrict__ xo, float4 *__restrict__ v,
+                                                const int32_t *__restrict__ mat, const uint32_t *__restrict__ nbr,
                                                 const uint32_t *__restrict__ cnt, uint32_t n, uint32_t stride,
-                                                PbfK q, const float *__restrict__ lam) {
+                                                PbfK q) {
     uint32_t i = slot_index();
     if (i >= n) return;
     if (mat[i] != VTP_MAT_FLUID) return;
     const float4 xi = x[i];
-    const float li = lam[i];
+    const float li = xl[i].w;
     const uint32_t nn = cnt[i] & ~kDeadBit;
     float ax = 0, ay = 0, az = 0;
     for_each_neighbor(
         nbr, stride, i, nn,
         [&](uint32_t j) {
-            float4 xj = x[j];
-            return make_float4(xj.x, xj.y, xj.z, (li + lam[j]) * (q.mass_weighted ? xj.w : 1.0f));
+            float4 e = ldg(xl, j);
+            if (MW) e.w = (li + e.w) * ldg_w(x, j);
+            else e.w = li + e.w;
+            return e;
         },
         [&](const float4 &xj) {
             float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -180,7 +189,7 @@ __global__ __launch_bounds__(256) void k_pbf_xsph(const float4 *__restrict__ x, 
     float ax = 0, ay = 0, az = 0;
     struct XV { float4 x, v; };
     for_each_neighbor(
-        nbr, stride, i, nn, [&](uint32_t j) { return XV{x[j], v[j]}; },
+        nbr, stride, i, nn, [&](uint32_t j) { return XV{ldg(x, j), ldg(v, j)}; },
         [&](const XV &d) {
             float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
             float r2 = dx * dx + dy * dy + dz * dz;
@@ -234,6 +243,7 @@ void common_reorder(Ctx &c, const float4 *xs, float4 *xd, float4 *xd2, const flo
 void pbf_initialize(Ctx &c) {
     const uint32_t n = c.n;
     c.lam.alloc(n);
+    c.xl.alloc(n);
     c.dens.alloc(n);
     c.err_c.alloc(n);
     VTP_HIP(hipMemsetAsync(c.lam.p, 0, c.lam.bytes(), c.stream));
@@ -276,11 +286,15 @@ static void pbf_enqueue_step(Ctx &c, PbfTapeStep *tape = nullptr) {
         float *errc = tape ? tape->C[it] : c.err_c.p;
         if (tape) VTP_HIP(hipMemcpyAsync(tape->x[it], cur, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
         hipLaunchKernelGGL(k_pbf_lambda, dim3(div_up(n, bl)), dim3(bl), 0, c.stream, cur, c.mat.p, u.nbr.p,
-                           u.cnt.p, n, u.stride, q, lam, c.dens.p, errc, tape ? tape->sg[it] : nullptr);
+                           u.cnt.p, n, u.stride, q, lam, c.dens.p, errc, tape ? tape->sg[it] : nullptr, c.xl.p);
         c.timer.end(c.stream);
         c.timer.begin(c.stream, "pbf_dx");
-        hipLaunchKernelGGL(k_pbf_dx, dim3(div_up(n, bd)), dim3(bd), 0, c.stream, cur, nxt, c.V[1].p, c.mat.p,
-                           u.nbr.p, u.cnt.p, n, u.stride, q, lam);
+        if (q.mass_weighted)
+            hipLaunchKernelGGL(k_pbf_dx<true>, dim3(div_up(n, bd)), dim3(bd), 0, c.stream, cur, c.xl.p, nxt,
+                               c.V[1].p, c.mat.p, u.nbr.p, u.cnt.p, n, u.stride, q);
+        else
+            hipLaunchKernelGGL(k_pbf_dx<false>, dim3(div_up(n, bd)), dim3(bd), 0, c.stream, cur, c.xl.p, nxt,
+                               c.V[1].p, c.mat.p, u.nbr.p, u.cnt.p, n, u.stride, q);
         c.timer.end(c.stream);
         std::swap(cur, nxt);
     }

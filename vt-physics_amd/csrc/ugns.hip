@@ -303,22 +303,22 @@ __global__ __launch_bounds__(kNbrBuildThreads) void k_build_neighbors(const floa
                     const uint32_t row = sbase + (uint32_t)Z * gxy + (uint32_t)Y * (uint32_t)g.gx;
                     uint32_t s = kEmpty, e = 0;
                     for (int X = X0; X <= X1; ++X) {
-                        uint32_t a = start[row + X];
+                        uint32_t a = ldg(start, row + X);
                         if (a != kEmpty) {
                             if (s == kEmpty) s = a;
-                            e = end[row + X];
+                            e = ldg(end, row + X);
                         }
                     }
                     if (s == kEmpty) continue;
                     uint32_t j = s;
                     for (; j + 4 <= e + 1; j += 4) {
-                        float4 a0 = x[j], a1 = x[j + 1], a2 = x[j + 2], a3 = x[j + 3];
+                        float4 a0 = ldg(x, j), a1 = ldg(x, j + 1), a2 = ldg(x, j + 2), a3 = ldg(x, j + 3);
                         test(j, a0);
                         test(j + 1, a1);
                         test(j + 2, a2);
                         test(j + 3, a3);
                     }
-                    for (; j <= e; ++j) test(j, x[j]);
+                    for (; j <= e; ++j) test(j, ldg(x, j));
                 }
             }
             if (c > cap) c = reference_scan(x, start, end, g, i, p, cx, cy, cz, sbase, dstride, drow, cap, dst);

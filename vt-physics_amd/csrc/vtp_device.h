@@ -157,30 +157,55 @@ __device__ __forceinline__ uint32_t xcd_block() {
 __device__ __forceinline__ uint32_t slot_index() { return xcd_block() * blockDim.x + threadIdx.x; }
 
 // ---- neighbour iteration ---------------------------------------------------
-// Walks the ELL list of slot i in batches of four: the four indices are
-// loaded first, then the four gathers are issued, then the four bodies run,
-// so each lane keeps several independent loads in flight instead of one
-// index -> gather dependency chain per neighbour.
+// Gathers through a 32-bit byte offset from a wave-uniform base: the loads
+// take the SGPR-base + VGPR-offset form, with no 64-bit address arithmetic per
+// neighbour.  Needs n * sizeof(T) < 4 GiB (vtp_initialize caps n at 2^28).
+template <typename T>
+__device__ __forceinline__ T ldg(const T *__restrict__ base, uint32_t j) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (uint64_t)(j * (uint32_t)sizeof(T)));
+}
+// Only the w component of a float4 element (a 4-byte load).
+__device__ __forceinline__ float ldg_w(const float4 *__restrict__ base, uint32_t j) {
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + (uint64_t)(j * 16u + 12u));
+}
+
+// Walks the ELL list of slot i (column t at nbr + t*stride) in batches of
+// four: the four gathers of a batch are issued together, and the indices of
+// the next batch are requested before the current batch is used, so the
+// index -> gather dependency of one batch overlaps the arithmetic of the
+// previous one.  The column base is wave-uniform (t is the same in every
+// lane), so index loads are one 256-byte line per wave and column.
 template <typename Load, typename Use>
 __device__ __forceinline__ void for_each_neighbor(const uint32_t *__restrict__ nbr, uint32_t stride, uint32_t i,
                                                   uint32_t nn, Load &&load, Use &&use) {
-    const uint32_t *p = nbr + i;
+    const char *const base = reinterpret_cast<const char *>(nbr);
+    const size_t step = (size_t)stride * sizeof(uint32_t);
+    const uint32_t off = i * (uint32_t)sizeof(uint32_t);
+    auto idx = [&](uint32_t t) { return *reinterpret_cast<const uint32_t *>(base + t * step + off); };
     uint32_t t = 0;
-    for (; t + 4 <= nn; t += 4) {
-        const uint32_t j0 = p[(size_t)t * stride];
-        const uint32_t j1 = p[(size_t)(t + 1) * stride];
-        const uint32_t j2 = p[(size_t)(t + 2) * stride];
-        const uint32_t j3 = p[(size_t)(t + 3) * stride];
-        auto d0 = load(j0);
-        auto d1 = load(j1);
-        auto d2 = load(j2);
-        auto d3 = load(j3);
-        use(d0);
-        use(d1);
-        use(d2);
-        use(d3);
+    if (nn >= 4) {
+        uint32_t j0 = idx(0), j1 = idx(1), j2 = idx(2), j3 = idx(3);
+        for (;;) {
+            auto d0 = load(j0);
+            auto d1 = load(j1);
+            auto d2 = load(j2);
+            auto d3 = load(j3);
+            const bool more = t + 8 <= nn;
+            if (more) {
+                j0 = idx(t + 4);
+                j1 = idx(t + 5);
+                j2 = idx(t + 6);
+                j3 = idx(t + 7);
+            }
+            use(d0);
+            use(d1);
+            use(d2);
+            use(d3);
+            t += 4;
+            if (!more) break;
+        }
     }
-    for (; t < nn; ++t) use(load(p[(size_t)t * stride]));
+    for (; t < nn; ++t) use(load(idx(t)));
 }
 
 // ---- wavefront reductions (wave64) ----------------------------------------

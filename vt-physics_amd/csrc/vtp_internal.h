@@ -234,6 +234,7 @@ struct Ctx {
 
     // PBF scratch (slot order)
     DevBuf<float> lam, dens, err_c;
+    DevBuf<float4> xl;          // (x, lambda) of the current constraint iteration
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     // PBF adjoint state (checkpoints and tapes), owned here
