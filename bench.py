@@ -63,6 +63,18 @@ def timed_steps(torch, ctx, steps):
     return time.perf_counter() - t0
 
 
+def stage_breakdown(ctx, steps):
+    """Per-stage device ms per tick from hipEvents on the context's stream
+    (eager launches; run after the timed region)."""
+    ctx.set_option("graph", 0)
+    ctx.set_option("stage_timing", 1)
+    ctx.reset_stage_times()
+    ctx.step(steps)
+    st = ctx.stage_times()
+    ctx.set_option("stage_timing", 0)
+    return {k: round(v[0] / steps, 4) for k, v in st.items()}
+
+
 def bench_dfsph(torch, vtp, scenes, steps, warmup):
     wl = scenes.dfsph_1m(seed=0)
     ctx, params = build_ctx(vtp, wl, graph=False)
@@ -70,6 +82,7 @@ def bench_dfsph(torch, vtp, scenes, steps, warmup):
     st0 = ctx.stats()
     el = timed_steps(torch, ctx, steps)
     st = ctx.stats()
+    stages = stage_breakdown(ctx, 2)
     ctx.close()
     it_div = (st["total_div_iterations"] - st0["total_div_iterations"]) / steps
     it_inc = (st["total_incomp_iterations"] - st0["total_incomp_iterations"]) / steps
@@ -77,7 +90,7 @@ def bench_dfsph(torch, vtp, scenes, steps, warmup):
             "value": round(wl.n_fluid * steps / el, 1), "unit": "particle-steps/s",
             "ms_per_step": round(el * 1e3 / steps, 4), "mean_div_iterations": it_div,
             "mean_incomp_iterations": it_inc, "steps": steps, "warmup": warmup,
-            "dead_particles": st["dead_particles"]}
+            "dead_particles": st["dead_particles"], "stages_ms_per_step": stages}
 
 
 def bench_immct(torch, vtp, scenes, steps, warmup):
@@ -89,9 +102,10 @@ def bench_immct(torch, vtp, scenes, steps, warmup):
     ctx.step(warmup)
     el = timed_steps(torch, ctx, steps)
     st = ctx.stats()
+    stages = stage_breakdown(ctx, 1)
     ctx.close()
     return {"workload": wl.name, "fluid_particles": wl.n_fluid, "boundary_particles": wl.n - wl.n_fluid,
-            "phases": 3, "value": round(wl.n_fluid * steps / el, 1), "unit": "particle-steps/s",
+            "stages_ms_per_step": stages, "phases": 3, "value": round(wl.n_fluid * steps / el, 1), "unit": "particle-steps/s",
             "ms_per_step": round(el * 1e3 / steps, 4), "steps": steps, "warmup": warmup,
             "mass_weighted_stress": True, "last_iterations": [st["div_iterations"], st["incomp_iterations"],
                                                               st["phase_iterations"]]}

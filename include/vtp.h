@@ -246,8 +246,11 @@ vtp_status vtp_grid_info(const vtp_ctx *ctx, uint32_t grid[3], uint32_t *cells);
 
 /* ---- execution options ---------------------------------------------------
  * Keys: "graph" (1 = capture the PBF substep in a hipGraph and replay it),
- * "autotune" (1 = time launch shapes at initialize and keep the fastest),
- * "block" (threads per block for the neighbour passes, 0 = default),
+ * "autotune" (1 = time launch shapes of the PBF/DFSPH neighbour passes, at
+ * initialize or immediately when already initialized, and keep the fastest;
+ * the state is restored afterwards),
+ * "block" (threads per block for the neighbour passes: 64, 128, 192 or 256;
+ * 0 = default 256),
  * "stage_timing" (1 = record hipEvents around every stage; see
  * vtp_stage_times), "solver_chunk" (iterations launched per host check). */
 vtp_status vtp_set_option(vtp_ctx *ctx, const char *key, double value);

@@ -103,3 +103,27 @@ def test_dfsph_iteration_cap_sets_crash():
     assert orc.stats()["incomp_iterations"] == 8
     with pytest.raises(vtp.VtpError, match="crashed"):
         ctx.step(1)
+
+
+def test_dfsph_cfl_step_closed_form():
+    """CFL extension: dt = min(dtMax, cfl * 2r / max|v_fluid|), measured at the
+    start of the tick.  An uncompressed cluster translating at 8 m/s with
+    gravity off feels no pressure or viscosity, so it moves by exactly u*dt."""
+    import json
+    r = 0.01
+    cfg = scenes.dfsph_config(radius=r, lb=(0, 0, 0), size=(1, 1, 1), dt=0.002, cfl=True)
+    cfg["DFSPH"]["Optional"] = {"enable": True, "gravity": [0.0, 0.0, 0.0]}
+    s = 1.1 * 2 * r
+    g = np.stack(np.meshgrid(*(np.arange(6) * s,) * 3, indexing="ij"), -1).reshape(-1, 3).astype(np.float32) + 0.3
+    u = np.array([8.0, 0.0, 0.0], np.float32)
+    p = vtp.params_from_json("DFSPH", json.dumps(cfg))
+    ctx = vtp.Context(p)
+    ctx.add_particles(g, vtp.MAT_FLUID, np.tile(u, (g.shape[0], 1)))
+    ctx.initialize()
+    ctx.step(1)
+    dt = np.float32(np.float32(0.4) * np.float32(2 * r) / np.float32(8.0))
+    st = ctx.stats()
+    assert st["dt"] == pytest.approx(float(dt), rel=1e-6)
+    assert st["sim_time"] == pytest.approx(float(dt), rel=1e-6)
+    np.testing.assert_allclose(ctx.read("pos"), g + u * dt, atol=2e-7)
+    ctx.close()

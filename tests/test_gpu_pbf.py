@@ -108,3 +108,17 @@ def test_pbf_repeat_runs_bit_identical():
     b.step(5)
     np.testing.assert_array_equal(a.read("pos"), b.read("pos"))
     np.testing.assert_array_equal(a.read("vel"), b.read("vel"))
+
+
+def test_pbf_autotune_keeps_results_bit_identical():
+    """The autotuner only changes workgroup sizes (per-particle results do not
+    depend on them) and restores the state it timed from."""
+    wl = scenes.cfg0_pbf(n_side=(16, 16, 16))
+    a, _ = _pair(wl)
+    b, _ = _pair(wl)
+    b.set_option("autotune", 1)
+    a.step(5)
+    b.step(5)
+    np.testing.assert_array_equal(a.read("pos"), b.read("pos"))
+    np.testing.assert_array_equal(a.read("vel"), b.read("vel"))
+    assert b.stats()["steps"] == a.stats()["steps"] == 5

@@ -319,6 +319,7 @@ extern "C" vtp_status vtp_initialize(vtp_ctx *ctx) {
         }
         read_status(c);
         c.initialized = true;
+        if (c.opt_autotune) autotune(c);   // requested before initialize
         return VTP_OK;
     });
 }
@@ -605,7 +606,7 @@ extern "C" vtp_status vtp_set_option(vtp_ctx *ctx, const char *key, double value
             pbf_capture_reset(c);
         } else if (k == "block") {
             uint32_t b = (uint32_t)value;
-            if (b != 0 && (b % 64 != 0 || b > 1024)) throw std::runtime_error("block must be a multiple of 64, <= 1024");
+            if (b != 0 && (b % 64 != 0 || b > 256)) throw std::runtime_error("block must be a multiple of 64, <= 256");
             c.opt_block = b ? b : 256;
             c.block_of.clear();
             pbf_capture_reset(c);

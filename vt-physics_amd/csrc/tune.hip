@@ -34,6 +34,8 @@ void autotune(Ctx &c) {
     VTP_HIP(hipMemcpyAsync(sv.p, c.V[0].p, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
     VTP_HIP(hipMemcpyAsync(so.p, c.orig.p, n * 4, hipMemcpyDeviceToDevice, c.stream));
     VTP_HIP(hipMemcpyAsync(sstat.p, c.status.p, kStWords * 4, hipMemcpyDeviceToDevice, c.stream));
+    const uint64_t total_div = c.total_div, total_inc = c.total_inc;
+    const uint32_t hint_div = c.iter_hint_div, hint_inc = c.iter_hint_inc;
     const bool timing_was = c.timer.enabled;
     const bool graph_was = c.opt_graph;
     c.opt_graph = false;
@@ -61,6 +63,10 @@ void autotune(Ctx &c) {
         launch_mat_from_orig(c);
     }
     for (const auto &kv : best) c.block_of[kv.first] = kv.second.second;
+    c.total_div = total_div;
+    c.total_inc = total_inc;
+    c.iter_hint_div = hint_div;
+    c.iter_hint_inc = hint_inc;
     c.timer.reset();
     c.timer.enabled = timing_was;
     c.opt_graph = graph_was;
