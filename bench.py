@@ -279,6 +279,10 @@ def main():
     passes = 3 if cells * wl.scenes() >= (1 << 16) else 2
     step_bytes = pbf_step_bytes(n, nf, cells * wl.scenes(), params.iteration_num, passes)
     st = ctx.stats()
+    import numpy as np
+    num = ctx.debug_neighbors()["neighbor_num"].astype(np.int64)   # self included, reference layout
+    nbr_counts = {"mean": round(float(num[num > 0].mean()), 2), "max": int(num.max()),
+                  "at_cap_fraction": round(float((num >= params.max_neighbor_num).mean()), 6)}
     ctx.close()
 
     extra = {}
@@ -311,6 +315,7 @@ def main():
         "stages_ms_per_step": {k: round(v[0] / prof_steps, 5) for k, v in stages.items()},
         "stage_total_ms_per_step": round(total / prof_steps, 4),
         "dead_particles": st["dead_particles"],
+        "neighbour_counts": nbr_counts,
         "dfsph_1m": extra.get("dfsph_1m"),
         "batched_64": extra.get("batched_64"),
         "adjoint": extra.get("adjoint"),

@@ -49,8 +49,8 @@ static DfK df_constants(const Ctx &c) {
 struct Loop {   // device pointers of one pressure-solve loop
     uint32_t *status;
     SolverScalars *sc;
-    const uint32_t *obj_cnt;   // particles per object
-    const int32_t *obj_fluid;  // 1 if the object is fluid
+    const double *obj_w;       // per object: 1/(particles * fluid objects) for fluid objects, else 0
+    double *partial;           // one residual partial sum per workgroup of the delta/kappa pass
     uint32_t nobj;
 };
 
@@ -136,17 +136,18 @@ __global__ __launch_bounds__(256) void k_df_precompute(const float4 *__restrict_
                                                        const uint32_t *__restrict__ nbr,
                                                        const uint32_t *__restrict__ cnt, uint32_t n,
                                                        uint32_t stride, DfK q, float *__restrict__ cr,
-                                                       float *__restrict__ alpha) {
+                                                       float *__restrict__ alpha, float4 *__restrict__ xv) {
     uint32_t i = slot_index();
     if (i >= n) return;
     const float4 vi = v[i];
+    const float4 xi = x[i];
     vadv[i] = vi;
+    xv[i] = make_float4(xi.x, xi.y, xi.z, vi.w);   // (x, V) for the v* passes of this tick
     if (mat[i] != VTP_MAT_FLUID) {
         cr[i] = 0.0f;
         alpha[i] = 0.0f;
         return;
     }
-    const float4 xi = x[i];
     const uint32_t w = cnt[i];
     const uint32_t nn = w & ~kDeadBit;
     float c = (w & kDeadBit) ? 0.0f : vi.w * q.k.w0;
@@ -176,7 +177,8 @@ __global__ void k_solve_begin(uint32_t *status, SolverScalars *sc, uint32_t nobj
         status[kStDone] = 0;
         status[kStIter] = 0;
     }
-    for (uint32_t o = threadIdx.x; o < nobj; o += blockDim.x) sc->obj_sum[o] = 0.0;
+    (void)sc;
+    (void)nobj;
 }
 
 // compute_delta_compression_ratio + update_delta_compression_ratio_from_vel_adv
@@ -197,9 +199,7 @@ __global__ __launch_bounds__(256) void k_df_delta_kappa(const float4 *__restrict
     if (lp.status[kStDone]) return;
     const uint32_t i = slot_index();
     const float dt = lp.sc->dt;
-    bool active = false;
     double contrib = 0.0;
-    uint32_t obj = 0;
     if (i < n) {
         if (mat[i] == VTP_MAT_FLUID) {
             const float4 xi = x[i];
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(256) void k_df_delta_kappa(const float4 *__restrict
             struct D { float4 x, a; };
             for_each_neighbor(
                 nbr, stride, i, nn,
-                [&](uint32_t j) { return D{ldg(x, j), make_float4(ldg(vadv, j).x, ldg(vadv, j).y, ldg(vadv, j).z, ldg_w(v, j))}; },
+                [&](uint32_t j) { return D{ldg(x, j), ldg(vadv, j)}; },   // vadv.w = V_j
                 [&](const D &e) {
                     float dx = xi.x - e.x.x, dy = xi.y - e.x.y, dz = xi.z - e.x.z;
                     float f = cubic_grad_factor(dx * dx + dy * dy + dz * dz, k);
@@ -220,30 +220,31 @@ __global__ __launch_bounds__(256) void k_df_delta_kappa(const float4 *__restrict
             float inv_dt = 1.0f / dt;
             delta[i] = dl;
             kappa[i] = dl / alpha[i] * (inv_dt * inv_dt) / v[i].w;
-            active = true;
-            contrib = dl;
-            obj = obj_of[i];
+            contrib = (double)dl * lp.obj_w[obj_of[i]];
         } else {
             kappa[i] = 0.0f;
         }
     }
-    // per-object FP64 sums: one atomic per (wavefront, object present)
-    const int lane = threadIdx.x & 63;
-    uint64_t pending = __ballot(active);
-    while (pending) {
-        int leader = __ffsll((unsigned long long)pending) - 1;
-        uint32_t o = __shfl(obj, leader, 64);
-        bool mine = active && obj == o;
-        uint64_t grp = __ballot(mine);
-        double s = wave_sum(mine ? contrib : 0.0);
-        if (lane == leader) atomicAdd(&lp.sc->obj_sum[o], s);
-        pending &= ~grp;
+    // residual = sum_i delta_i w_obj(i) (the mean over fluid objects of the
+    // per-object mean of delta): one FP64 partial per workgroup, reduced in a
+    // fixed order by k_df_decide, so the residual is deterministic and no
+    // atomic touches a shared address
+    __shared__ double s_part[kMaxWavesPerBlock];
+    const double wsum = wave_sum(contrib);
+    const uint32_t w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0) s_part[w] = wsum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (uint32_t q = 0; q < nw; ++q) t += s_part[q];
+        lp.partial[blockIdx.x] = t;
     }
 }
 
 // vf_update_vel_adv_from_kappa_* (DFSPHCudaImpl.cu:198-220, 369-391):
 //   v*_i -= dt V_i V_j / m_i (kappa_i + kappa_j) gradW_ij over all neighbours.
 __global__ __launch_bounds__(256) void k_df_update_vadv(const float4 *__restrict__ x, const float4 *__restrict__ v,
+                                                        const float4 *__restrict__ xv,
                                                         float4 *__restrict__ vadv, const int32_t *__restrict__ mat,
                                                         const uint32_t *__restrict__ nbr,
                                                         const uint32_t *__restrict__ cnt, uint32_t n,
@@ -260,7 +261,7 @@ __global__ __launch_bounds__(256) void k_df_update_vadv(const float4 *__restrict
     float ax = 0, ay = 0, az = 0;
     struct D { float4 x; float k; };
     for_each_neighbor(
-        nbr, stride, i, nn, [&](uint32_t j) { return D{make_float4(ldg(x, j).x, ldg(x, j).y, ldg(x, j).z, ldg_w(v, j)), ldg(kappa, j)}; },
+        nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(xv, j), ldg(kappa, j)}; },   // (x_j, V_j), kappa_j
         [&](const D &e) {
             float dx = xi.x - e.x.x, dy = xi.y - e.x.y, dz = xi.z - e.x.z;
             float s = cubic_grad_factor(dx * dx + dy * dy + dz * dz, k) * e.x.w * (ki + e.k);
@@ -278,22 +279,25 @@ __global__ __launch_bounds__(256) void k_df_update_vadv(const float4 *__restrict
 
 // Loop rule of vfsph_div / vfsph_incomp (DFSPHCudaImpl.cu:458-502): after the
 // v* update, stop when the residual of this iteration is below the threshold
-// or more than `cap` iterations ran (then the crash bit is raised).
-__global__ void k_df_decide(Loop lp, float thr, uint32_t cap, uint32_t iter_word, uint32_t crash_bit, int which) {
-    if (threadIdx.x != 0 || lp.status[kStDone]) return;
-    uint32_t iter = lp.status[kStIter] + 1;
-    lp.status[kStIter] = iter;
-    double total = 0.0;
-    int nf = 0;
-    for (uint32_t o = 0; o < lp.nobj; ++o) {
-        if (lp.obj_fluid[o]) {
-            uint32_t c = lp.obj_cnt[o];
-            total += c ? lp.sc->obj_sum[o] / (double)c : 0.0;
-            ++nf;
-        }
-        lp.sc->obj_sum[o] = 0.0;
+// or more than `cap` iterations ran (then the crash bit is raised).  One
+// workgroup sums the per-workgroup partials of the delta/kappa pass in a
+// fixed order.
+__global__ __launch_bounds__(256) void k_df_decide(Loop lp, uint32_t nparts, float thr, uint32_t cap,
+                                                   uint32_t iter_word, uint32_t crash_bit, int which) {
+    if (lp.status[kStDone]) return;
+    __shared__ double s_red[256];
+    double t = 0.0;
+    for (uint32_t k = threadIdx.x; k < nparts; k += blockDim.x) t += lp.partial[k];
+    s_red[threadIdx.x] = t;
+    __syncthreads();
+    for (uint32_t off = blockDim.x / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) s_red[threadIdx.x] += s_red[threadIdx.x + off];
+        __syncthreads();
     }
-    double res = nf ? total / nf : 0.0;
+    if (threadIdx.x != 0) return;
+    const uint32_t iter = lp.status[kStIter] + 1;
+    lp.status[kStIter] = iter;
+    const double res = s_red[0];
     if (which == 0) lp.sc->div_residual = res; else lp.sc->inc_residual = res;
     if (res < (double)thr || iter > cap) {
         lp.status[kStDone] = 1;
@@ -448,7 +452,7 @@ void launch_step_begin(Ctx &c);
 void launch_hash(Ctx &c, const float4 *x);
 
 static Loop make_loop(Ctx &c) {
-    return Loop{c.status.p, c.scal.p, c.obj_cnt.p, c.obj_fluid.p, (uint32_t)c.objects.size()};
+    return Loop{c.status.p, c.scal.p, c.obj_w.p, c.partial.p, (uint32_t)c.objects.size()};
 }
 
 // Pressure solve with device-side termination; the host checks the flag once
@@ -476,10 +480,12 @@ void df_pressure_solve(Ctx &c, const float4 *x, const float4 *v, int which) {
                                c.kappa.p, lp);
             c.timer.end(c.stream);
             c.timer.begin(c.stream, name_up);
-            hipLaunchKernelGGL(k_df_update_vadv, dim3(div_up(n, bu)), dim3(bu), 0, c.stream, x, v, c.vadv.p, c.mat.p,
+            hipLaunchKernelGGL(k_df_update_vadv, dim3(div_up(n, bu)), dim3(bu), 0, c.stream, x, v, c.xv.p, c.vadv.p,
+                               c.mat.p,
                                u.nbr.p, u.cnt.p, n, u.stride, c.kern, c.kappa.p, lp);
             c.timer.end(c.stream);
-            hipLaunchKernelGGL(k_df_decide, dim3(1), dim3(64), 0, c.stream, lp, thr, cap, word, bit, which);
+            hipLaunchKernelGGL(k_df_decide, dim3(1), dim3(256), 0, c.stream, lp, div_up(n, bd), thr, cap, word,
+                               bit, which);
         }
         read_status(c);
         if (c.h_status[kStDone] || launched > cap) break;
@@ -503,6 +509,7 @@ void dfsph_allocate_common(Ctx &c) {
     c.vadv.alloc(n);
     c.acc.alloc(n);
     c.normal.alloc(n);
+    c.xv.alloc(n);
     VTP_HIP(hipMemsetAsync(c.cr.p, 0, c.cr.bytes(), c.stream));
     VTP_HIP(hipMemsetAsync(c.alpha.p, 0, c.alpha.bytes(), c.stream));
     VTP_HIP(hipMemsetAsync(c.delta.p, 0, c.delta.bytes(), c.stream));
@@ -521,6 +528,15 @@ void dfsph_allocate_common(Ctx &c) {
     c.obj_fluid.alloc(no);
     VTP_HIP(hipMemcpyAsync(c.obj_cnt.p, oc.data(), no * 4, hipMemcpyHostToDevice, c.stream));
     VTP_HIP(hipMemcpyAsync(c.obj_fluid.p, of.data(), no * 4, hipMemcpyHostToDevice, c.stream));
+    // residual weights (CommonFunc.cpp:10-21: mean over fluid objects of each object's mean)
+    uint32_t nfo = 0;
+    for (uint32_t o = 0; o < no; ++o) nfo += of[o];
+    std::vector<double> ow(no, 0.0);
+    for (uint32_t o = 0; o < no; ++o)
+        if (of[o] && oc[o]) ow[o] = 1.0 / ((double)oc[o] * (double)nfo);
+    c.obj_w.alloc(no);
+    VTP_HIP(hipMemcpyAsync(c.obj_w.p, ow.data(), no * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    c.partial.alloc(div_up(n, 64));   // smallest workgroup of the delta/kappa pass
     SolverScalars s{};
     s.dt = c.p.time_step;
     VTP_HIP(hipMemcpyAsync(c.scal.p, &s, sizeof s, hipMemcpyHostToDevice, c.stream));
@@ -571,7 +587,7 @@ void dfsph_step(Ctx &c) {
     const uint32_t bp = c.block("df_precompute");
     c.timer.begin(c.stream, "df_precompute");
     hipLaunchKernelGGL(k_df_precompute, dim3(div_up(n, bp)), dim3(bp), 0, c.stream, x, v, c.vadv.p, c.mat.p, u.nbr.p,
-                       u.cnt.p, n, u.stride, q, c.cr.p, c.alpha.p);
+                       u.cnt.p, n, u.stride, q, c.cr.p, c.alpha.p, c.xv.p);
     c.timer.end(c.stream);
     df_pressure_solve(c, x, v, 0);
     c.timer.begin(c.stream, "df_apply_pressure");

@@ -21,9 +21,9 @@
 namespace vtp {
 
 constexpr int kRsThreads = 256;
-constexpr int kRsRounds = 16;                   // keys per lane per tile
-constexpr int kRsWaveChunk = 64 * kRsRounds;    // 1024 keys per wavefront
-constexpr int kRsTile = kRsThreads * kRsRounds; // 4096 keys per workgroup
+constexpr int kRsRounds = 4;                    // keys per lane per tile
+constexpr int kRsWaveChunk = 64 * kRsRounds;    // 256 keys per wavefront
+constexpr int kRsTile = kRsThreads * kRsRounds; // 1024 keys per workgroup (1.3k tiles at 1.3M keys)
 // Neighbour build: 128 threads, each with a private LDS column of `cap`
 // words (30 KB at cap 59, five workgroups per CU); larger caps write the
 // lists straight to global memory.
@@ -91,8 +91,8 @@ __global__ __launch_bounds__(kRsThreads) void k_radix_spine(uint32_t *__restrict
     if (tid == kRsThreads - 1) dtot[blockIdx.x] = s[tid];
 }
 
-// Stable scatter of one tile.  Wavefront w owns keys [w*1024, (w+1)*1024) of
-// the tile, processed in 16 rounds of 64 in index order, so the order
+// Stable scatter of one tile.  Wavefront w owns keys [w*256, (w+1)*256) of
+// the tile, processed in 4 rounds of 64 in index order, so the order
 // (wave, round, lane) equals the input order and ranks are stable.
 __global__ __launch_bounds__(kRsThreads) void k_radix_scatter(const uint32_t *__restrict__ kin,
                                                               const uint32_t *__restrict__ vin,

@@ -148,6 +148,7 @@ __device__ __forceinline__ float surface_tension_c(float r, float h, float c_sig
 // keeps a particle's neighbours (and the neighbours of its neighbours) in the
 // same L2 instead of spreading every neighbourhood over all eight.
 constexpr uint32_t kXcds = 8;
+constexpr uint32_t kMaxWavesPerBlock = 16;   // 1024-thread workgroups at most
 __device__ __forceinline__ uint32_t xcd_block() {
     const uint32_t b = blockIdx.x, nb = gridDim.x;
     const uint32_t x = b % kXcds, k = b / kXcds;   // XCD, rank inside that XCD

@@ -246,8 +246,11 @@ struct Ctx {
     DevBuf<float> rest_attach, vis_attach;   // static per-particle properties, attach order
     DevBuf<uint32_t> obj_cnt;                // particles per attach object
     DevBuf<int32_t> obj_fluid;               // 1 if the object is fluid
+    DevBuf<double> obj_w;                    // residual weight per object
+    DevBuf<double> partial;                  // residual partial per workgroup
     uint32_t iter_hint_div = 1, iter_hint_inc = 1;
     DevBuf<float4> vadv, acc, normal;
+    DevBuf<float4> xv;   // (x, V) of the current tick, gathered by the v* passes
     // IMM (phase-major: [k*stride + slot])
     uint32_t P = 0;
     DevBuf<float> frac[2], frac_in, frac_out;
