@@ -252,7 +252,8 @@ vtp_status vtp_grid_info(const vtp_ctx *ctx, uint32_t grid[3], uint32_t *cells);
  * "block" (threads per block for the neighbour passes: 64, 128, 192 or 256;
  * 0 = default 256),
  * "stage_timing" (1 = record hipEvents around every stage; see
- * vtp_stage_times), "solver_chunk" (iterations launched per host check). */
+ * vtp_stage_times), "solver_chunk" (iterations launched per host check),
+ * "lanes" (1 or 4 lanes per particle in the PBF neighbour passes; default 1). */
 vtp_status vtp_set_option(vtp_ctx *ctx, const char *key, double value);
 
 /* Per-stage device time (ms, summed over launches) and launch counts,

@@ -614,6 +614,10 @@ extern "C" vtp_status vtp_set_option(vtp_ctx *ctx, const char *key, double value
             c.timer.enabled = value != 0.0;
         } else if (k == "solver_chunk") {
             c.opt_chunk = (uint32_t)value;
+        } else if (k == "lanes") {
+            if (value != 1.0 && value != 4.0) throw std::runtime_error("lanes must be 1 or 4");
+            c.opt_lanes = (uint32_t)value;
+            pbf_capture_reset(c);
         } else {
             throw std::runtime_error("unknown option '" + k + "'");
         }

@@ -66,6 +66,7 @@ __global__ void k_pbf_predict_hash(float4 *__restrict__ x, float4 *__restrict__ 
 //   S_i = sum_j |gradW_ij/rho0|^2 + |sum_j gradW_ij / rho0|^2,
 //   lambda_i = -C_i / (S_i + 1e-6)          (mass-free gradient, as the reference).
 // Boundary lambda is defined as 0 (the reference never writes it).
+template <bool MW, uint32_t L>
 __global__ __launch_bounds__(256) void k_pbf_lambda(const float4 *__restrict__ x,
                                                     const int32_t *__restrict__ mat,
                                                     const uint32_t *__restrict__ nbr,
@@ -73,10 +74,11 @@ __global__ __launch_bounds__(256) void k_pbf_lambda(const float4 *__restrict__ x
                                                     uint32_t stride, PbfK q, float *__restrict__ lam,
                                                     float *__restrict__ dens, float *__restrict__ errc,
                                                     float4 *__restrict__ sg, float4 *__restrict__ xl) {
-    uint32_t i = slot_index();
+    const uint32_t g = slot_index(), i = g / L, sub = g % L;   // L lanes per particle
     if (i >= n) return;
     const float4 xi = x[i];
     if (mat[i] != VTP_MAT_FLUID) {
+        if (sub != 0) return;
         // slots are permuted every tick, so non-fluid slots are written too
         lam[i] = 0.0f;
         dens[i] = 0.0f;
@@ -87,23 +89,28 @@ __global__ __launch_bounds__(256) void k_pbf_lambda(const float4 *__restrict__ x
     }
     const uint32_t w = cnt[i];
     const uint32_t nn = w & ~kDeadBit;
-    float rho = (w & kDeadBit) ? 0.0f : xi.w * q.k.w0;
+    float rho = ((w & kDeadBit) || sub != 0) ? 0.0f : xi.w * q.k.w0;
     float gsx = 0, gsy = 0, gsz = 0, s2 = 0;
-    for_each_neighbor(
-        nbr, stride, i, nn, [&](uint32_t j) { return ldg(x, j); },
+    for_each_neighbor_coop<L>(
+        nbr, stride, i, nn, sub, [&](uint32_t j) { return ldg(x, j); },
         [&](const float4 &xj) {
             float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
             float r2 = dx * dx + dy * dy + dz * dz;
             float W, f;
             cubic_w_grad(r2, q.k, W, f);
             rho = fmaf(xj.w, W, rho);
-            if (q.mass_weighted) f *= xj.w;
-            float ax = dx * f, ay = dy * f, az = dz * f;
-            gsx += ax;
-            gsy += ay;
-            gsz += az;
-            s2 += ax * ax + ay * ay + az * az;
+            if (MW) f *= xj.w;
+            gsx = fmaf(dx, f, gsx);
+            gsy = fmaf(dy, f, gsy);
+            gsz = fmaf(dz, f, gsz);
+            s2 = fmaf(f * f, r2, s2);   // |f r_vec|^2
         });
+    rho = lane_sum<L>(rho);
+    gsx = lane_sum<L>(gsx);
+    gsy = lane_sum<L>(gsy);
+    gsz = lane_sum<L>(gsz);
+    s2 = lane_sum<L>(s2);
+    if (sub != 0) return;
     rho = fmaxf(rho, q.rho0);
     // rho/rho0 - 1 with a rounded division, as the reference (PBFCudaImpl.cu:92):
     // a clamped particle gets exactly C = 0 (a fused rho*(1/rho0) - 1 would
@@ -125,21 +132,21 @@ __global__ __launch_bounds__(256) void k_pbf_lambda(const float4 *__restrict__ x
 // Neighbours come from xl = (x, lambda) written by the lambda pass: one
 // 16-byte gather per neighbour.  MW (extension) weights the pair by m_j, read
 // from the position buffer.
-template <bool MW>
+template <bool MW, uint32_t L>
 __global__ __launch_bounds__(256) void k_pbf_dx(const float4 *__restrict__ x, const float4 *__restrict__ xl,
                                                 float4 *__restrict__ xo, float4 *__restrict__ v,
                                                 const int32_t *__restrict__ mat, const uint32_t *__restrict__ nbr,
                                                 const uint32_t *__restrict__ cnt, uint32_t n, uint32_t stride,
                                                 PbfK q) {
-    uint32_t i = slot_index();
+    const uint32_t g = slot_index(), i = g / L, sub = g % L;
     if (i >= n) return;
     if (mat[i] != VTP_MAT_FLUID) return;
     const float4 xi = x[i];
     const float li = xl[i].w;
     const uint32_t nn = cnt[i] & ~kDeadBit;
     float ax = 0, ay = 0, az = 0;
-    for_each_neighbor(
-        nbr, stride, i, nn,
+    for_each_neighbor_coop<L>(
+        nbr, stride, i, nn, sub,
         [&](uint32_t j) {
             float4 e = ldg(xl, j);
             if (MW) e.w = (li + e.w) * ldg_w(x, j);
@@ -154,9 +161,10 @@ __global__ __launch_bounds__(256) void k_pbf_dx(const float4 *__restrict__ x, co
             ay = fmaf(dy, s, ay);
             az = fmaf(dz, s, az);
         });
-    ax *= q.inv_rho0;
-    ay *= q.inv_rho0;
-    az *= q.inv_rho0;
+    ax = lane_sum<L>(ax) * q.inv_rho0;
+    ay = lane_sum<L>(ay) * q.inv_rho0;
+    az = lane_sum<L>(az) * q.inv_rho0;
+    if (sub != 0) return;
     xo[i] = make_float4(xi.x + ax, xi.y + ay, xi.z + az, xi.w);
     float4 vi = v[i];
     vi.x += ax * q.inv_dt;
@@ -170,17 +178,19 @@ __global__ __launch_bounds__(256) void k_pbf_dx(const float4 *__restrict__ x, co
 //   x_i <- x_i + dx_i,  v_i <- v_i + dx_i / dt.
 // Writes every slot of the output buffers (boundary slots are copied), so the
 // state buffers X[0]/V[0] are complete after the pass.
+template <uint32_t L>
 __global__ __launch_bounds__(256) void k_pbf_xsph(const float4 *__restrict__ x, float4 *__restrict__ xo,
                                                   const float4 *__restrict__ v, float4 *__restrict__ vo,
                                                   const int32_t *__restrict__ mat,
                                                   const uint32_t *__restrict__ nbr,
                                                   const uint32_t *__restrict__ cnt, uint32_t n,
                                                   uint32_t stride, PbfK q) {
-    uint32_t i = slot_index();
+    const uint32_t g = slot_index(), i = g / L, sub = g % L;
     if (i >= n) return;
     const float4 xi = x[i];
     const float4 vi = v[i];
     if (mat[i] != VTP_MAT_FLUID) {
+        if (sub != 0) return;
         xo[i] = xi;
         vo[i] = vi;
         return;
@@ -188,8 +198,8 @@ __global__ __launch_bounds__(256) void k_pbf_xsph(const float4 *__restrict__ x, 
     const uint32_t nn = cnt[i] & ~kDeadBit;
     float ax = 0, ay = 0, az = 0;
     struct XV { float4 x, v; };
-    for_each_neighbor(
-        nbr, stride, i, nn, [&](uint32_t j) { return XV{ldg(x, j), ldg(v, j)}; },
+    for_each_neighbor_coop<L>(
+        nbr, stride, i, nn, sub, [&](uint32_t j) { return XV{ldg(x, j), ldg(v, j)}; },
         [&](const XV &d) {
             float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
             float r2 = dx * dx + dy * dy + dz * dz;
@@ -198,6 +208,10 @@ __global__ __launch_bounds__(256) void k_pbf_xsph(const float4 *__restrict__ x, 
             ay = fmaf(d.v.y - vi.y, s, ay);
             az = fmaf(d.v.z - vi.z, s, az);
         });
+    ax = lane_sum<L>(ax);
+    ay = lane_sum<L>(ay);
+    az = lane_sum<L>(az);
+    if (sub != 0) return;
     float c = q.xsph_k * q.dt;
     float dx = ax * c, dy = ay * c, dz = az * c;
     xo[i] = make_float4(xi.x + dx, xi.y + dy, xi.z + dz, xi.w);
@@ -280,21 +294,28 @@ static void pbf_enqueue_step(Ctx &c, PbfTapeStep *tape = nullptr) {
     }
     float4 *cur = c.X[1].p, *nxt = c.X[2].p;
     const uint32_t bl = c.block("pbf_lambda"), bd = c.block("pbf_dx"), bx = c.block("pbf_xsph");
+    const uint32_t L = c.opt_lanes;
     for (uint32_t it = 0; it < c.p.iteration_num; ++it) {
         c.timer.begin(c.stream, "pbf_lambda");
         float *lam = tape ? tape->lam[it] : c.lam.p;
         float *errc = tape ? tape->C[it] : c.err_c.p;
+        float4 *sg = tape ? tape->sg[it] : nullptr;
         if (tape) VTP_HIP(hipMemcpyAsync(tape->x[it], cur, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
-        hipLaunchKernelGGL(k_pbf_lambda, dim3(div_up(n, bl)), dim3(bl), 0, c.stream, cur, c.mat.p, u.nbr.p,
-                           u.cnt.p, n, u.stride, q, lam, c.dens.p, errc, tape ? tape->sg[it] : nullptr, c.xl.p);
+        const dim3 gl(div_up(n * L, bl)), gd(div_up(n * L, bd));
+#define VTP_LAMBDA(MW, LL)                                                                                     \
+    hipLaunchKernelGGL((k_pbf_lambda<MW, LL>), gl, dim3(bl), 0, c.stream, cur, c.mat.p, u.nbr.p, u.cnt.p, n,    \
+                       u.stride, q, lam, c.dens.p, errc, sg, c.xl.p)
+#define VTP_DX(MW, LL)                                                                                         \
+    hipLaunchKernelGGL((k_pbf_dx<MW, LL>), gd, dim3(bd), 0, c.stream, cur, c.xl.p, nxt, c.V[1].p, c.mat.p,    \
+                       u.nbr.p, u.cnt.p, n, u.stride, q)
+        if (q.mass_weighted) { if (L == 4) VTP_LAMBDA(true, 4); else VTP_LAMBDA(true, 1); }
+        else { if (L == 4) VTP_LAMBDA(false, 4); else VTP_LAMBDA(false, 1); }
         c.timer.end(c.stream);
         c.timer.begin(c.stream, "pbf_dx");
-        if (q.mass_weighted)
-            hipLaunchKernelGGL(k_pbf_dx<true>, dim3(div_up(n, bd)), dim3(bd), 0, c.stream, cur, c.xl.p, nxt,
-                               c.V[1].p, c.mat.p, u.nbr.p, u.cnt.p, n, u.stride, q);
-        else
-            hipLaunchKernelGGL(k_pbf_dx<false>, dim3(div_up(n, bd)), dim3(bd), 0, c.stream, cur, c.xl.p, nxt,
-                               c.V[1].p, c.mat.p, u.nbr.p, u.cnt.p, n, u.stride, q);
+        if (q.mass_weighted) { if (L == 4) VTP_DX(true, 4); else VTP_DX(true, 1); }
+        else { if (L == 4) VTP_DX(false, 4); else VTP_DX(false, 1); }
+#undef VTP_LAMBDA
+#undef VTP_DX
         c.timer.end(c.stream);
         std::swap(cur, nxt);
     }
@@ -304,8 +325,12 @@ static void pbf_enqueue_step(Ctx &c, PbfTapeStep *tape = nullptr) {
         VTP_HIP(hipMemcpyAsync(tape->v_last, c.V[1].p, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
     }
     c.timer.begin(c.stream, "pbf_xsph");
-    hipLaunchKernelGGL(k_pbf_xsph, dim3(div_up(n, bx)), dim3(bx), 0, c.stream, cur, c.X[0].p, c.V[1].p,
-                       c.V[0].p, c.mat.p, u.nbr.p, u.cnt.p, n, u.stride, q);
+    if (L == 4)
+        hipLaunchKernelGGL(k_pbf_xsph<4>, dim3(div_up(n * 4, bx)), dim3(bx), 0, c.stream, cur, c.X[0].p, c.V[1].p,
+                           c.V[0].p, c.mat.p, u.nbr.p, u.cnt.p, n, u.stride, q);
+    else
+        hipLaunchKernelGGL(k_pbf_xsph<1>, dim3(div_up(n, bx)), dim3(bx), 0, c.stream, cur, c.X[0].p, c.V[1].p,
+                           c.V[0].p, c.mat.p, u.nbr.p, u.cnt.p, n, u.stride, q);
     c.timer.end(c.stream);
 }
 

@@ -16,6 +16,8 @@
 //  * neighbour lists hold slot indices in an ELL layout nbr[t*stride + slot]
 //    (t-major: lane i of a wavefront reads word i, fully coalesced) without
 //    the self entry; the reference layout is rebuilt on request.
+#include <algorithm>
+
 #include "vtp_internal.h"
 
 namespace vtp {
@@ -24,11 +26,10 @@ constexpr int kRsThreads = 256;
 constexpr int kRsRounds = 4;                    // keys per lane per tile
 constexpr int kRsWaveChunk = 64 * kRsRounds;    // 256 keys per wavefront
 constexpr int kRsTile = kRsThreads * kRsRounds; // 1024 keys per workgroup (1.3k tiles at 1.3M keys)
-// Neighbour build: 128 threads, each with a private LDS column of `cap`
-// words (30 KB at cap 59, five workgroups per CU); larger caps write the
-// lists straight to global memory.
+// Neighbour build: 128 threads, each with a private LDS column of up to
+// kNbrStage words.
 constexpr uint32_t kNbrBuildThreads = 128;
-constexpr size_t kNbrBuildMaxLds = 64 * 1024;
+constexpr uint32_t kNbrStage = 32;   // list entries per thread staged in LDS (16 KB per workgroup)
 
 __global__ void k_fill_u32(uint32_t *a, uint32_t n, uint32_t v) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -211,10 +212,10 @@ __global__ void k_cell_fixup(const uint32_t *__restrict__ key, const uint32_t *_
 // cells in the reference's offset order (x outermost, z innermost,
 // UniformGridNeighborSearch.cpp:33-60), every other particle of a cell in
 // slot order that passes 0 < |x_i - x_j| <= h, capped at `cap` entries.
+template <typename Put>
 __device__ uint32_t reference_scan(const float4 *__restrict__ x, const uint32_t *__restrict__ start,
                                    const uint32_t *__restrict__ end, const Grid &g, uint32_t i, float3 p, int cx,
-                                   int cy, int cz, uint32_t sbase, uint32_t stride, uint32_t row, uint32_t cap,
-                                   uint32_t *__restrict__ nbr) {
+                                   int cy, int cz, uint32_t sbase, uint32_t cap, Put &&put) {
     const uint32_t gxy = (uint32_t)(g.gx * g.gy);
     uint32_t c = 0;
     for (int ox = -1; ox <= 1; ++ox) {
@@ -234,7 +235,7 @@ __device__ uint32_t reference_scan(const float4 *__restrict__ x, const uint32_t 
                     if (j == i) continue;
                     float d2 = dist2_exact(p, xyz(x[j]));
                     if (is_neighbor(d2, g.nbr_d2_max) && c < cap) {
-                        nbr[(size_t)c * stride + row] = j;
+                        put(c, j);
                         ++c;
                     }
                 }
@@ -253,11 +254,13 @@ __device__ uint32_t reference_scan(const float4 *__restrict__ x, const uint32_t 
 // (The order inside a list only matters for truncation; the reference-layout
 // export re-sorts every list into the reference order.)
 //
-// kStaged: each thread assembles its list in a private LDS column
-// (s_list[c * blockDim + tid]) and the wave then writes the ELL columns out
-// one at a time, so every store of the wave covers 64 consecutive slots of
-// one column.  Written straight to HBM, the lanes' appends land in `cap`
-// different columns at once and the L2 evicts most lines half-written.
+// kStaged: each thread assembles the first kNbrStage entries of its list in
+// a private LDS column (s_list[c * blockDim + tid]) and the wave then writes
+// those ELL columns out one at a time, so every store of the wave covers 64
+// consecutive slots of one column.  Written straight to HBM, the lanes'
+// appends land in `cap` different columns at once and the L2 evicts most
+// lines half-written.  Entries past kNbrStage (dense regions only) are
+// written directly; the smaller LDS footprint keeps occupancy up.
 template <bool kStaged>
 __global__ __launch_bounds__(kNbrBuildThreads) void k_build_neighbors(const float4 *__restrict__ x,
                                                                       const uint32_t *__restrict__ key,
@@ -268,9 +271,11 @@ __global__ __launch_bounds__(kNbrBuildThreads) void k_build_neighbors(const floa
                                                                       uint32_t *__restrict__ cnt) {
     extern __shared__ uint32_t s_list[];
     const uint32_t i = slot_index();
-    uint32_t *const dst = kStaged ? s_list : nbr;
-    const uint32_t dstride = kStaged ? blockDim.x : stride;
-    const uint32_t drow = kStaged ? threadIdx.x : i;
+    const uint32_t stage = kStaged ? min(cap, kNbrStage) : 0u;
+    auto put = [&](uint32_t c, uint32_t j) {
+        if (c < stage) s_list[c * blockDim.x + threadIdx.x] = j;
+        else nbr[(size_t)c * stride + i] = j;
+    };
     uint32_t c = 0;
     if (i < n) {
         const uint32_t k = key[i];
@@ -290,7 +295,7 @@ __global__ __launch_bounds__(kNbrBuildThreads) void k_build_neighbors(const floa
             auto test = [&](uint32_t j, const float4 &xj) {
                 float d2 = dist2_exact(p, xyz(xj));
                 if (j != i && is_neighbor(d2, g.nbr_d2_max)) {
-                    if (c < cap) dst[(size_t)c * dstride + drow] = j;
+                    if (c < cap) put(c, j);
                     ++c;
                 }
             };
@@ -321,13 +326,13 @@ __global__ __launch_bounds__(kNbrBuildThreads) void k_build_neighbors(const floa
                     for (; j <= e; ++j) test(j, ldg(x, j));
                 }
             }
-            if (c > cap) c = reference_scan(x, start, end, g, i, p, cx, cy, cz, sbase, dstride, drow, cap, dst);
+            if (c > cap) c = reference_scan(x, start, end, g, i, p, cx, cy, cz, sbase, cap, put);
             cnt[i] = c;
         }
     }
     if (kStaged) {
         // every lane reads back only its own LDS column: no barrier needed
-        const uint32_t cmax = wave_max_u32(c);
+        const uint32_t cmax = min(wave_max_u32(c), stage);
         for (uint32_t col = 0; col < cmax; ++col)
             if (col < c) nbr[(size_t)col * stride + i] = s_list[col * blockDim.x + threadIdx.x];
     }
@@ -443,13 +448,9 @@ void Ugns::sort_and_rank(const uint32_t *orig_old, hipStream_t s, StageTimer &tm
 
 void Ugns::build_neighbors(const float4 *x, hipStream_t s, StageTimer &tm) {
     tm.begin(s, "ugns_neighbors");
-    const size_t lds = (size_t)cap * kNbrBuildThreads * sizeof(uint32_t);
-    if (lds <= kNbrBuildMaxLds)
-        hipLaunchKernelGGL(k_build_neighbors<true>, dim3(blocks_for(n, kNbrBuildThreads)), dim3(kNbrBuildThreads),
-                           lds, s, x, key[sorted].p, start.p, end.p, g, n, stride, cap, nbr.p, cnt.p);
-    else
-        hipLaunchKernelGGL(k_build_neighbors<false>, dim3(blocks_for(n, kNbrBuildThreads)), dim3(kNbrBuildThreads),
-                           0, s, x, key[sorted].p, start.p, end.p, g, n, stride, cap, nbr.p, cnt.p);
+    const size_t lds = (size_t)std::min(cap, kNbrStage) * kNbrBuildThreads * sizeof(uint32_t);
+    hipLaunchKernelGGL(k_build_neighbors<true>, dim3(blocks_for(n, kNbrBuildThreads)), dim3(kNbrBuildThreads), lds,
+                       s, x, key[sorted].p, start.p, end.p, g, n, stride, cap, nbr.p, cnt.p);
     tm.end(s);
 }
 

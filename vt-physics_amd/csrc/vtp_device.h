@@ -87,49 +87,51 @@ __host__ inline Cubic make_cubic(float h) {
     return c;
 }
 
-// W(r) from r^2.  q <= 0.5: sigma(6q^3 - 6q^2 + 1); q <= 1: 2 sigma (1-q)^3.
-// Both branches are evaluated and selected (no divergent branch in the
-// neighbour loops).  r = r2 * rsqrt(r2) (one transcendental; a correctly
+// The branches are evaluated and selected (no divergent branch in the
+// neighbour loops), sharing t = max(1 - q, 0): the outer branch is then zero
+// past q = 1 by itself (W and its derivative vanish at q = 1), so the q > 1
+// test costs nothing.  r = r2 * rsqrt(r2) (one transcendental; a correctly
 // rounded sqrt costs a dozen instructions and changes W by < 1e-7 relative).
+//   W:    q <= 0.5: sigma(1 + q^2 (6q - 6));   q <= 1: 2 sigma (1-q)^3
+//   grad: q <= 0.5: 6q(3q - 2) sigma/h;         q <= 1: -6 (1-q)^2 sigma/h, times r_vec/r
+// Squared-distance bound of the reference's q < 1e-6 gradient cut-off.
+__device__ __forceinline__ float cubic_r2_lo(const Cubic &k) { return (1e-6f * k.h) * (1e-6f * k.h); }
+
 __device__ __forceinline__ float cubic_w(float r2, const Cubic &k) {
-    float r = r2 > 0.0f ? r2 * __frsqrt_rn(r2) : 0.0f;
-    float q = r * k.inv_h;
-    float t = 1.0f - q;
-    float wi = fmaf(6.0f * q * q, q - 1.0f, 1.0f);
-    float wo = 2.0f * t * t * t;
-    float w = (q <= 0.5f) ? wi : wo;
-    return q <= 1.0f ? k.sigma * w : 0.0f;
+    const float r = r2 > 0.0f ? r2 * __frsqrt_rn(r2) : 0.0f;
+    const float q = r * k.inv_h;
+    const float t = fmaxf(1.0f - q, 0.0f);
+    const float wi = fmaf(q * q, fmaf(q, 6.0f, -6.0f), 1.0f);
+    const float wo = 2.0f * (t * t) * t;
+    return k.sigma * (q <= 0.5f ? wi : wo);
 }
 
-// Scalar factor f such that gradW(r_vec) = f * r_vec:
-// f = sigma/(r h) * (q <= 0.5 ? 6(3q^2 - 2q) : -6(1-q)^2), zero for q < 1e-6 or q > 1.
+// Scalar factor f such that gradW(r_vec) = f * r_vec (zero for q < 1e-6).
 __device__ __forceinline__ float cubic_grad_factor(float r2, const Cubic &k) {
-    float rinv = __frsqrt_rn(r2);
-    float r = r2 * rinv;
-    float q = r * k.inv_h;
-    float t = 1.0f - q;
-    float gi = 6.0f * q * fmaf(3.0f, q, -2.0f);
-    float go = -6.0f * t * t;
-    float g = (q <= 0.5f) ? gi : go;
-    float f = k.grad_scale * g * rinv;
-    return (q >= 1e-6f && q <= 1.0f) ? f : 0.0f;  // NaN (r2 == 0) -> 0
+    const float rinv = __frsqrt_rn(r2);
+    const float q = (r2 * rinv) * k.inv_h;
+    const float t = fmaxf(1.0f - q, 0.0f);
+    const float gi = q * fmaf(q, 18.0f, -12.0f);
+    const float go = -6.0f * (t * t);
+    const float f = k.grad_scale * (q <= 0.5f ? gi : go) * rinv;
+    return r2 >= cubic_r2_lo(k) ? f : 0.0f;   // also r2 == 0 (rinv = inf)
 }
 
 // W and the gradient factor together (one rsqrt).
 __device__ __forceinline__ void cubic_w_grad(float r2, const Cubic &k, float &w, float &f) {
-    float rinv = __frsqrt_rn(r2);
-    float r = r2 * rinv;
-    float q = r * k.inv_h;
-    float t = 1.0f - q;
-    bool inner = q <= 0.5f;
-    float wi = fmaf(6.0f * q * q, q - 1.0f, 1.0f);
-    float wo = 2.0f * t * t * t;
-    float ww = inner ? wi : wo;
-    w = q <= 1.0f ? k.sigma * ww : 0.0f;
-    float gi = 6.0f * q * fmaf(3.0f, q, -2.0f);
-    float go = -6.0f * t * t;
-    float g = inner ? gi : go;
-    f = (q >= 1e-6f && q <= 1.0f) ? k.grad_scale * g * rinv : 0.0f;
+    const float rinv = __frsqrt_rn(r2);
+    const float q = (r2 * rinv) * k.inv_h;
+    const float t = fmaxf(1.0f - q, 0.0f);
+    const float t2 = t * t;
+    const bool inner = q <= 0.5f;
+    const float wi = fmaf(q * q, fmaf(q, 6.0f, -6.0f), 1.0f);
+    const float wo = 2.0f * t2 * t;
+    const float gi = q * fmaf(q, 18.0f, -12.0f);
+    const float go = -6.0f * t2;
+    const bool ok = r2 >= cubic_r2_lo(k);
+    // r2 == 0 only for a coincident pair: W(0) = sigma, gradient 0
+    w = ok ? k.sigma * (inner ? wi : wo) : (r2 > 0.0f ? k.sigma * wi : k.sigma);
+    f = ok ? k.grad_scale * (inner ? gi : go) * rinv : 0.0f;
 }
 
 // Akinci cohesion spline C(r) (KernelFunctions.cuh:58-69), c = 32/(pi h^9).
@@ -170,12 +172,14 @@ __device__ __forceinline__ float ldg_w(const float4 *__restrict__ base, uint32_t
     return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + (uint64_t)(j * 16u + 12u));
 }
 
-// Walks the ELL list of slot i (column t at nbr + t*stride) in batches of
-// four: the four gathers of a batch are issued together, and the indices of
-// the next batch are requested before the current batch is used, so the
-// index -> gather dependency of one batch overlaps the arithmetic of the
-// previous one.  The column base is wave-uniform (t is the same in every
-// lane), so index loads are one 256-byte line per wave and column.
+// Walks the ELL list of slot i (column t at nbr + t*stride) in batches: the
+// gathers of a batch are issued together, and the indices of the next batch
+// are requested before the current batch is used, so the index -> gather
+// dependency of one batch overlaps the arithmetic of the previous one.  A
+// batch is 8 neighbours when a neighbour's data fits in 32 bytes and 4 when
+// it is larger (register pressure).  The column base is wave-uniform (t is
+// the same in every lane), so index loads are one 256-byte line per wave and
+// column.
 template <typename Load, typename Use>
 __device__ __forceinline__ void for_each_neighbor(const uint32_t *__restrict__ nbr, uint32_t stride, uint32_t i,
                                                   uint32_t nn, Load &&load, Use &&use) {
@@ -183,30 +187,79 @@ __device__ __forceinline__ void for_each_neighbor(const uint32_t *__restrict__ n
     const size_t step = (size_t)stride * sizeof(uint32_t);
     const uint32_t off = i * (uint32_t)sizeof(uint32_t);
     auto idx = [&](uint32_t t) { return *reinterpret_cast<const uint32_t *>(base + t * step + off); };
+    using D = decltype(load(0u));
+    constexpr uint32_t kNbrBatch = sizeof(D) > 32 ? 4 : 8;
     uint32_t t = 0;
-    if (nn >= 4) {
-        uint32_t j0 = idx(0), j1 = idx(1), j2 = idx(2), j3 = idx(3);
+    if (nn >= kNbrBatch) {
+        uint32_t j[kNbrBatch];
+#pragma unroll
+        for (uint32_t b = 0; b < kNbrBatch; ++b) j[b] = idx(b);
         for (;;) {
-            auto d0 = load(j0);
-            auto d1 = load(j1);
-            auto d2 = load(j2);
-            auto d3 = load(j3);
-            const bool more = t + 8 <= nn;
+            D d[kNbrBatch];
+#pragma unroll
+            for (uint32_t b = 0; b < kNbrBatch; ++b) d[b] = load(j[b]);
+            const bool more = t + 2 * kNbrBatch <= nn;
             if (more) {
-                j0 = idx(t + 4);
-                j1 = idx(t + 5);
-                j2 = idx(t + 6);
-                j3 = idx(t + 7);
+#pragma unroll
+                for (uint32_t b = 0; b < kNbrBatch; ++b) j[b] = idx(t + kNbrBatch + b);
             }
-            use(d0);
-            use(d1);
-            use(d2);
-            use(d3);
-            t += 4;
+#pragma unroll
+            for (uint32_t b = 0; b < kNbrBatch; ++b) use(d[b]);
+            t += kNbrBatch;
             if (!more) break;
         }
     }
+    // remainder: batches of four, then singles
+    for (; t + 4 <= nn; t += 4) {
+        const uint32_t j0 = idx(t), j1 = idx(t + 1), j2 = idx(t + 2), j3 = idx(t + 3);
+        D d0 = load(j0), d1 = load(j1), d2 = load(j2), d3 = load(j3);
+        use(d0);
+        use(d1);
+        use(d2);
+        use(d3);
+    }
     for (; t < nn; ++t) use(load(idx(t)));
+}
+
+// Cooperative walk: L lanes share one particle, lane `sub` taking columns
+// sub, sub + L, sub + 2L, ...  In one gather instruction the L lanes of a
+// particle then read consecutive entries of its list (mostly neighbouring
+// slots of one cell row), so a wave touches far fewer cache lines than with
+// one particle per lane; the partial sums are combined with lane_sum<L>.
+template <uint32_t L, typename Load, typename Use>
+__device__ __forceinline__ void for_each_neighbor_coop(const uint32_t *__restrict__ nbr, uint32_t stride,
+                                                       uint32_t i, uint32_t nn, uint32_t sub, Load &&load,
+                                                       Use &&use) {
+    if (L == 1) {
+        for_each_neighbor(nbr, stride, i, nn, load, use);
+        return;
+    }
+    const char *const base = reinterpret_cast<const char *>(nbr);
+    const size_t step = (size_t)stride * sizeof(uint32_t);
+    const uint32_t off = i * (uint32_t)sizeof(uint32_t);
+    auto idx = [&](uint32_t t) { return *reinterpret_cast<const uint32_t *>(base + t * step + off); };
+    using D = decltype(load(0u));
+    constexpr uint32_t B = sizeof(D) > 32 ? 2 : 4;   // neighbours in flight per lane
+    uint32_t t = sub;
+    for (; t + (B - 1) * L < nn; t += B * L) {
+        uint32_t j[B];
+#pragma unroll
+        for (uint32_t b = 0; b < B; ++b) j[b] = idx(t + b * L);
+        D d[B];
+#pragma unroll
+        for (uint32_t b = 0; b < B; ++b) d[b] = load(j[b]);
+#pragma unroll
+        for (uint32_t b = 0; b < B; ++b) use(d[b]);
+    }
+    for (; t < nn; t += L) use(load(idx(t)));
+}
+
+// Sum over the L lanes (L = 1, 2 or 4, aligned groups) sharing one particle.
+template <uint32_t L>
+__device__ __forceinline__ float lane_sum(float v) {
+    if (L >= 2) v += __shfl_xor(v, 1, 64);
+    if (L >= 4) v += __shfl_xor(v, 2, 64);
+    return v;
 }
 
 // ---- wavefront reductions (wave64) ----------------------------------------

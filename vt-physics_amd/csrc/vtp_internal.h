@@ -215,6 +215,7 @@ struct Ctx {
     bool opt_autotune = false;
     uint32_t opt_block = 256;
     uint32_t opt_chunk = 0;   // 0 = adaptive
+    uint32_t opt_lanes = 1;   // lanes per particle in the PBF neighbour passes (1 or 4)
     StageTimer timer;
 
     // attach-order tables (static)
