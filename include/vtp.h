@@ -244,6 +244,53 @@ vtp_status vtp_debug_neighbors(vtp_ctx *ctx, uint32_t *neighbors,
  * *cells = cells per scene. */
 vtp_status vtp_grid_info(const vtp_ctx *ctx, uint32_t grid[3], uint32_t *cells);
 
+/* ---- checkpoint / resume --------------------------------------------------
+ * No counterpart in the reference: its exporter writes positions and colour
+ * only (ExportUtil.cpp:28-88), so a run cannot be resumed.  A state image is
+ * this header followed by the carried state in the solver's current slot
+ * order, each array packed (no padding), little-endian:
+ *   uint32 orig[n]                 attach id of each slot
+ *   float  x[n][4]                 position, w = mass
+ *   float  v[n][4]                 velocity, w = volume
+ *   IMM / IMM-CT only:
+ *   float  vol_frac[P][n]          volume fraction per phase
+ *   float  vel_phase[P][n][4]      phase velocity (w unused)
+ *   float  rest_density[n]
+ *   float  color[n][4]             (w unused)
+ *   IMM-CT only:
+ *   float  Q[9][n]                 conformation tensor, row-major components
+ * Loading into a context built with the same solver type and the same
+ * attached objects (count, material, scene and size of each) continues the run
+ * bit for bit: the ticks after vtp_load_state equal the ticks the saved
+ * context would have done. */
+typedef struct vtp_state_header {
+    char magic[8];              /* "VTPSTATE"                                   */
+    uint32_t version;           /* 1                                            */
+    int32_t solver;             /* vtp_solver_type                              */
+    uint32_t n;                 /* particles                                    */
+    uint32_t phases;            /* P (0 for PBF and DFSPH)                      */
+    uint32_t objects;           /* attached objects                             */
+    uint32_t crash_flags;       /* VTP_CRASH_* bits at save time                */
+    uint64_t layout_hash;       /* FNV-1a of solver, n, P and every object's
+                                   (start, end, scene, material)               */
+    uint64_t steps;
+    double sim_time;
+    uint64_t total_div_iterations;
+    uint64_t total_incomp_iterations;
+    uint32_t iter_hint_div, iter_hint_incomp;   /* iterations of the last solves */
+    uint64_t payload_bytes;     /* bytes that follow the header                 */
+} vtp_state_header;
+
+/* Bytes of the state image of an initialized context (header included). */
+vtp_status vtp_state_size(vtp_ctx *ctx, size_t *bytes);
+/* Writes the state image to `host_dst`; `bytes` must be at least
+ * vtp_state_size.  Synchronises the context's stream. */
+vtp_status vtp_save_state(vtp_ctx *ctx, void *host_dst, size_t bytes);
+/* Replaces the carried state of an initialized context by a state image.
+ * VTP_EINVAL (nothing changed) when the magic, version, solver type, layout
+ * hash or size do not match.  Neighbour lists are rebuilt by the next tick. */
+vtp_status vtp_load_state(vtp_ctx *ctx, const void *host_src, size_t bytes);
+
 /* ---- execution options ---------------------------------------------------
  * Keys: "graph" (1 = capture the PBF substep in a hipGraph and replay it),
  * "autotune" (1 = time launch shapes of the PBF/DFSPH neighbour passes, at

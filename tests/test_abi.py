@@ -123,3 +123,56 @@ def test_cfl_extension():
     cfg = vtp.scenes.dfsph_config(cfl=True)
     p = vtp.params_from_json("DFSPH", json.dumps(cfg))
     assert p.cfl_enable == 1 and np.isclose(p.cfl_number, 0.4) and np.isclose(p.dt_max, 0.002)
+
+
+def test_state_header_layout_matches_c(tmp_path):
+    """ctypes VtpStateHeader has the C struct's size and field offsets."""
+    import subprocess
+    fields = [f for f, _ in _abi.VtpStateHeader._fields_]
+    prog = tmp_path / "hdr.c"
+    prog.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "vtp.h"\nint main(void) {\n'
+                    '  printf("%zu\\n", sizeof(vtp_state_header));\n'
+                    + "".join(f'  printf("%zu\\n", offsetof(vtp_state_header, {f}));\n' for f in fields)
+                    + "  return 0;\n}\n")
+    exe = tmp_path / "hdr"
+    subprocess.run(["gcc", "-std=c11", "-I", str(ROOT / "include"), str(prog), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    want = [C.sizeof(_abi.VtpStateHeader)] + [getattr(_abi.VtpStateHeader, f).offset for f in fields]
+    assert got == want
+
+
+def _image(solver, n, P):
+    h = _abi.VtpStateHeader()
+    h.magic = b"VTPSTATE"
+    h.version, h.solver, h.n, h.phases, h.steps = 1, solver, n, P, 7
+    rng = np.random.default_rng(0)
+    parts = []
+    for name, dt, shape in _abi.state_sections(h):
+        a = rng.permutation(n).astype(np.uint32) if name == "orig" else rng.random(shape).astype(dt)
+        parts.append((name, a))
+    h.payload_bytes = sum(a.nbytes for _, a in parts)
+    return bytes(h) + b"".join(a.tobytes() for _, a in parts), dict(parts)
+
+
+@pytest.mark.parametrize("solver,P,names", [
+    (_abi.PBF, 0, ["orig", "x", "v"]),
+    (_abi.IMM, 3, ["orig", "x", "v", "vol_frac", "vel_phase", "rest_density", "color"]),
+    (_abi.IMMCT, 2, ["orig", "x", "v", "vol_frac", "vel_phase", "rest_density", "color", "Q"]),
+])
+def test_parse_state_sections(solver, P, names):
+    data, parts = _image(solver, 37, P)
+    h, arrays = vtp.parse_state(data)
+    assert h.n == 37 and h.steps == 7 and list(arrays) == names
+    for k in names:
+        assert np.array_equal(arrays[k], parts[k])
+    with pytest.raises(ValueError):
+        vtp.parse_state(data[:-1])
+    with pytest.raises(ValueError):
+        vtp.parse_state(b"XXXXXXXX" + data[8:])
+
+
+def test_state_calls_reject_null_context(lib):
+    size = C.c_size_t()
+    assert lib.vtp_state_size(None, C.byref(size)) == _abi.VTP_EINVAL
+    assert lib.vtp_save_state(None, None, 0) == _abi.VTP_EINVAL
+    assert lib.vtp_load_state(None, None, 0) == _abi.VTP_EINVAL

@@ -336,6 +336,7 @@ static vtp_status check_after_steps(Ctx &c) {
 }
 
 void vtp::solver_tick(Ctx &c) {
+    c.lists_stale = false;   // every tick rebuilds the lists first
     switch (c.p.solver) {
         case VTP_PBF: pbf_step(c); break;
         case VTP_DFSPH: dfsph_step(c); break;
@@ -556,6 +557,8 @@ extern "C" vtp_status vtp_debug_neighbors(vtp_ctx *ctx, uint32_t *neighbors, uin
                                           uint32_t *sorted_ids, uint32_t *cell_start, uint32_t *cell_end) {
     CTX_GUARD(ctx);
     if (!c.initialized) return ctx_fail(c, VTP_ESTATE, "vtp_debug_neighbors: not initialized");
+    if (c.lists_stale)
+        return ctx_fail(c, VTP_ESTATE, "vtp_debug_neighbors: lists belong to a replaced state; tick first");
     return guarded(c, [&]() -> vtp_status {
         const uint32_t n = c.n, M = c.p.max_neighbor_num;
         if (neighbors || neighbor_num) {

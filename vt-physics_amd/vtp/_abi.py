@@ -100,6 +100,57 @@ class VtpStepStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad_"}
 
 
+class VtpStateHeader(C.Structure):
+    """vtp_state_header: the fixed header of a state image."""
+    _fields_ = [
+        ("magic", C.c_char * 8),
+        ("version", C.c_uint32),
+        ("solver", C.c_int32),
+        ("n", C.c_uint32),
+        ("phases", C.c_uint32),
+        ("objects", C.c_uint32),
+        ("crash_flags", C.c_uint32),
+        ("layout_hash", C.c_uint64),
+        ("steps", C.c_uint64),
+        ("sim_time", C.c_double),
+        ("total_div_iterations", C.c_uint64),
+        ("total_incomp_iterations", C.c_uint64),
+        ("iter_hint_div", C.c_uint32),
+        ("iter_hint_incomp", C.c_uint32),
+        ("payload_bytes", C.c_uint64),
+    ]
+
+
+def state_sections(header: VtpStateHeader) -> list[tuple[str, np.dtype, tuple]]:
+    """(name, dtype, shape) of each array that follows the header, in order."""
+    n, P = header.n, header.phases
+    out = [("orig", np.uint32, (n,)), ("x", np.float32, (n, 4)), ("v", np.float32, (n, 4))]
+    if header.solver in (IMM, IMMCT):
+        out += [("vol_frac", np.float32, (P, n)), ("vel_phase", np.float32, (P, n, 4)),
+                ("rest_density", np.float32, (n,)), ("color", np.float32, (n, 4))]
+    if header.solver == IMMCT:
+        out.append(("Q", np.float32, (9, n)))
+    return out
+
+
+def parse_state(data: bytes) -> tuple[VtpStateHeader, dict]:
+    """Split a state image into its header and named arrays (views, no copy)."""
+    hs = C.sizeof(VtpStateHeader)
+    if len(data) < hs:
+        raise ValueError("state image shorter than its header")
+    h = VtpStateHeader.from_buffer_copy(data[:hs])
+    if h.magic != b"VTPSTATE":
+        raise ValueError("not a state image")
+    arrays, off = {}, hs
+    for name, dt, shape in state_sections(h):
+        cnt = int(np.prod(shape))
+        arrays[name] = np.frombuffer(data, dtype=dt, count=cnt, offset=off).reshape(shape)
+        off += cnt * np.dtype(dt).itemsize
+    if off != len(data) or off - hs != h.payload_bytes:
+        raise ValueError("state image size does not match its header")
+    return h, arrays
+
+
 # Every symbol include/vtp.h declares, with (restype, argtypes).
 _P = C.c_void_p
 _F = C.POINTER(C.c_float)
@@ -123,6 +174,9 @@ SIGNATURES = {
     "vtp_stats": (C.c_int, [_P, C.POINTER(VtpStepStats)]),
     "vtp_debug_neighbors": (C.c_int, [_P, _U, _U, _U, _U, _U]),
     "vtp_grid_info": (C.c_int, [_P, _U, _U]),
+    "vtp_state_size": (C.c_int, [_P, C.POINTER(C.c_size_t)]),
+    "vtp_save_state": (C.c_int, [_P, C.c_void_p, C.c_size_t]),
+    "vtp_load_state": (C.c_int, [_P, C.c_void_p, C.c_size_t]),
     "vtp_set_option": (C.c_int, [_P, C.c_char_p, C.c_double]),
     "vtp_stage_times": (C.c_int32, [_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_double),
                                     C.POINTER(C.c_uint64), C.c_int32]),
@@ -315,6 +369,24 @@ class Context:
         self._check(self.lib.vtp_debug_neighbors(self.h, uptr(lists), uptr(num), uptr(srt), uptr(cs), uptr(ce)))
         return {"neighbors": lists.reshape(n, M), "neighbor_num": num, "sorted": srt,
                 "cell_start": cs, "cell_end": ce}
+
+    def save_state(self, path: os.PathLike | str | None = None) -> bytes:
+        """State image of the context (vtp_save_state); also written to `path` if given."""
+        size = C.c_size_t()
+        self._check(self.lib.vtp_state_size(self.h, C.byref(size)))
+        buf = C.create_string_buffer(size.value)
+        self._check(self.lib.vtp_save_state(self.h, buf, size.value))
+        data = buf.raw
+        if path is not None:
+            Path(path).write_bytes(data)
+        return data
+
+    def load_state(self, data: bytes | os.PathLike | str):
+        """Resume from a state image (bytes, or the path of a file save_state wrote)."""
+        if not isinstance(data, (bytes, bytearray)):
+            data = Path(data).read_bytes()
+        buf = C.create_string_buffer(bytes(data), len(data))
+        self._check(self.lib.vtp_load_state(self.h, buf, len(data)))
 
     def stage_times(self):
         names = C.create_string_buffer(8192)
