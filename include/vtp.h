@@ -278,6 +278,10 @@ typedef struct vtp_state_header {
     uint64_t total_div_iterations;
     uint64_t total_incomp_iterations;
     uint32_t iter_hint_div, iter_hint_incomp;   /* iterations of the last solves */
+    uint32_t nbr_at_cap;        /* particles whose list reached the cap in the
+                                   last neighbour build (selects the next
+                                   build's walk, which orders the lists)      */
+    uint32_t reserved;          /* 0                                            */
     uint64_t payload_bytes;     /* bytes that follow the header                 */
 } vtp_state_header;
 
@@ -300,7 +304,12 @@ vtp_status vtp_load_state(vtp_ctx *ctx, const void *host_src, size_t bytes);
  * 0 = default 256),
  * "stage_timing" (1 = record hipEvents around every stage; see
  * vtp_stage_times), "solver_chunk" (iterations launched per host check),
- * "lanes" (1 or 4 lanes per particle in the PBF neighbour passes; default 1). */
+ * "lanes" (1 or 4 lanes per particle in the PBF neighbour passes; default 1),
+ * "neighbor_walk" (0 = adaptive, default: the list build walks cell rows and
+ * switches to the reference's cell order once many lists reach the cap;
+ * 1 = always rows; 2 = always the reference's order.  The neighbour sets are
+ * the reference's either way; the order inside a list, and so the FP32
+ * rounding of neighbour sums, depends on the walk). */
 vtp_status vtp_set_option(vtp_ctx *ctx, const char *key, double value);
 
 /* Per-stage device time (ms, summed over launches) and launch counts,

@@ -131,3 +131,29 @@ def test_load_rejects_other_layouts():
         dfsph.load_state(image)
     for c in (a, small, dfsph):
         c.close()
+
+
+def test_resume_keeps_the_adaptive_list_walk():
+    # Most lists of this scene reach the cap, so after the first tick every
+    # build takes the reference walk, whose list order (and FP32 rounding)
+    # differs from the row walk.  The image carries the cap count, so the
+    # resumed context takes the same walks as the saved one.
+    print("[state] adaptive walk", flush=True)
+    rng = np.random.default_rng(13)
+    u = rng.uniform(0.0, 1.0, size=(3000, 3))
+    pos = np.column_stack([0.3 * u[:, 0] ** 3, 0.3 * u[:, 1], 0.3 * u[:, 2]]).astype(np.float32)
+    wl = scenes.Workload("grad", "PBF", scenes.pbf_config(lb=(0, 0, 0), size=(0.3, 0.3, 0.3), max_nbr=40),
+                         [scenes.Obj(vtp.MAT_FLUID, pos)])
+    a = _context(wl, "PBF")
+    a.step(3)
+    image = a.save_state()
+    h, _ = vtp.parse_state(image)
+    assert h.nbr_at_cap > h.n // 32
+    a.step(3)
+    want = _snapshot(a, CARRIED["PBF"])
+    b = _context(wl, "PBF")
+    b.load_state(image)
+    b.step(3)
+    _assert_same(want, _snapshot(b, CARRIED["PBF"]))
+    a.close()
+    b.close()

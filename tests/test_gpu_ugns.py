@@ -10,9 +10,11 @@ from vtp import scenes
 pytestmark = pytest.mark.gpu
 
 
-def _compare_ugns(wl, params=None):
+def _compare_ugns(wl, params=None, walk=None):
     params = params or vtp.params_from_json(wl.solver, wl.config_json())
     ctx = vtp.Context(params)
+    if walk is not None:
+        ctx.set_option("neighbor_walk", walk)
     for o in wl.objects:
         ctx.add_particles(o.pos, o.mat, o.vel, o.vol_frac, o.scene)
     ctx.initialize()
@@ -39,22 +41,24 @@ def test_ugns_cfg0_positions_bit_exact():
     assert g["neighbor_num"].max() <= 60
 
 
-def test_ugns_random_points_bit_exact():
+@pytest.mark.parametrize("walk", [1, 2])
+def test_ugns_random_points_bit_exact(walk):
     rng = np.random.default_rng(3)
     pos = rng.uniform(0.0, 0.3, size=(5000, 3)).astype(np.float32)
     wl = scenes.Workload("rand", "PBF", scenes.pbf_config(lb=(0, 0, 0), size=(0.3, 0.3, 0.3)),
                          [scenes.Obj(vtp.MAT_FLUID, pos)])
-    _compare_ugns(wl)
+    _compare_ugns(wl, walk=walk)
 
 
-def test_ugns_dense_cap_truncation():
+@pytest.mark.parametrize("walk", [1, 2])
+def test_ugns_dense_cap_truncation(walk):
     # 400 particles inside one cell: lists are truncated at maxNeighborNum in
     # the reference traversal order.
     rng = np.random.default_rng(5)
     pos = rng.uniform(0.1, 0.13, size=(400, 3)).astype(np.float32)
     wl = scenes.Workload("dense", "PBF", scenes.pbf_config(lb=(0, 0, 0), size=(0.4, 0.4, 0.4), max_nbr=20),
                          [scenes.Obj(vtp.MAT_FLUID, pos)])
-    g, _ = _compare_ugns(wl)
+    g, _ = _compare_ugns(wl, walk=walk)
     assert (g["neighbor_num"] == 20).all()
 
 
@@ -80,3 +84,54 @@ def test_ugns_batched_scenes_do_not_mix():
     for p in range(0, lists.shape[0], 97):
         nb = lists[p, : g["neighbor_num"][p]]
         assert (sc[nb] == sc[p]).all()
+
+
+@pytest.mark.parametrize("walk", [1, 2])
+def test_ugns_cap_cut_inside_the_x_plus_one_cells(walk):
+    # Cell (1,2,2) holds 10 particles, cell (2,2,2) 60, all within h of each
+    # other.  A particle of the first cell finds 9 neighbours in the x-1 and x
+    # cells and 60 in the x+1 cell: its list (cap 19) keeps the first 10 of
+    # the x+1 cell in the reference order.  A particle of the second cell
+    # finds 69 in its x-1 and x cells: the reference traversal decides.
+    rng = np.random.default_rng(11)
+    a = np.column_stack([rng.uniform(0.070, 0.079, 10), rng.uniform(0.10, 0.11, (10, 2))])
+    b = np.column_stack([rng.uniform(0.081, 0.090, 60), rng.uniform(0.10, 0.11, (60, 2))])
+    pos = np.concatenate([a, b]).astype(np.float32)
+    wl = scenes.Workload("xcut", "PBF", scenes.pbf_config(lb=(0, 0, 0), size=(0.4, 0.4, 0.4), max_nbr=20),
+                         [scenes.Obj(vtp.MAT_FLUID, pos)])
+    g, _ = _compare_ugns(wl, walk=walk)
+    assert (g["neighbor_num"] == 20).all()
+    first = g["neighbors"][0, 1:20]
+    assert (first < 10).sum() == 9 and (first >= 10).sum() == 10
+
+
+@pytest.mark.parametrize("walk", [1, 2])
+def test_ugns_density_gradient_bit_exact(walk):
+    # lattice-free points whose density rises 27-fold towards x = 0, so lists
+    # range from a few entries to far past the cap
+    rng = np.random.default_rng(13)
+    u = rng.uniform(0.0, 1.0, size=(3000, 3))
+    pos = np.column_stack([0.3 * u[:, 0] ** 3, 0.3 * u[:, 1], 0.3 * u[:, 2]]).astype(np.float32)
+    wl = scenes.Workload("grad", "PBF", scenes.pbf_config(lb=(0, 0, 0), size=(0.3, 0.3, 0.3), max_nbr=40),
+                         [scenes.Obj(vtp.MAT_FLUID, pos)])
+    g, _ = _compare_ugns(wl, walk=walk)
+    num = g["neighbor_num"]
+    assert (num == 40).mean() > 0.2 and (num < 40).mean() > 0.2
+
+
+@pytest.mark.parametrize("walk", [1, 2])
+def test_ugns_cap_reached_in_the_x_minus_one_and_x_cells(walk):
+    # 20 particles in cell (1,2,2) and 10 in cell (2,2,2), all within h.  With
+    # cap 19, a particle of the first cell fills its list from its own cell
+    # exactly (19 = cap); one of the second cell meets 29 in its x-1 and x
+    # cells and is rebuilt by the reference traversal.
+    rng = np.random.default_rng(17)
+    a = np.column_stack([rng.uniform(0.075, 0.0799, 20), rng.uniform(0.10, 0.105, (20, 2))])
+    b = np.column_stack([rng.uniform(0.0801, 0.085, 10), rng.uniform(0.10, 0.105, (10, 2))])
+    pos = np.concatenate([a, b]).astype(np.float32)
+    wl = scenes.Workload("xfront", "PBF", scenes.pbf_config(lb=(0, 0, 0), size=(0.4, 0.4, 0.4), max_nbr=20),
+                         [scenes.Obj(vtp.MAT_FLUID, pos)])
+    g, _ = _compare_ugns(wl, walk=walk)
+    assert (g["neighbor_num"] == 20).all()
+    assert (g["neighbors"][0, 1:20] < 20).all()
+    assert (g["neighbors"][25, 1:20] < 20).all()

@@ -281,6 +281,7 @@ struct TapeBufs {
 struct Checkpoint {
     DevBuf<float4> x, v;
     DevBuf<uint32_t> orig;
+    DevBuf<uint32_t> at_cap;   // neighbour-cap count of the last build (selects the next build's walk)
 };
 
 struct AdjointState {
@@ -309,6 +310,7 @@ static void save_ckpt(Ctx &c, Checkpoint &k) {
     VTP_HIP(hipMemcpyAsync(k.x.p, c.X[0].p, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
     VTP_HIP(hipMemcpyAsync(k.v.p, c.V[0].p, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
     VTP_HIP(hipMemcpyAsync(k.orig.p, c.orig.p, n * 4, hipMemcpyDeviceToDevice, c.stream));
+    VTP_HIP(hipMemcpyAsync(k.at_cap.p, c.status.p + kStNbrAtCap, 4, hipMemcpyDeviceToDevice, c.stream));
 }
 
 static void load_ckpt(Ctx &c, Checkpoint &k) {
@@ -316,6 +318,7 @@ static void load_ckpt(Ctx &c, Checkpoint &k) {
     VTP_HIP(hipMemcpyAsync(c.X[0].p, k.x.p, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
     VTP_HIP(hipMemcpyAsync(c.V[0].p, k.v.p, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
     VTP_HIP(hipMemcpyAsync(c.orig.p, k.orig.p, (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream));
+    VTP_HIP(hipMemcpyAsync(c.status.p + kStNbrAtCap, k.at_cap.p, 4, hipMemcpyDeviceToDevice, c.stream));
     hipLaunchKernelGGL(k_adj_mat_from_orig, dim3(div_up(n, 256)), dim3(256), 0, c.stream, c.orig.p,
                        c.mat_attach.p, c.scene_attach.p, n, c.mat.p, c.scene.p);
 }
@@ -418,6 +421,7 @@ extern "C" vtp_status vtp_pbf_rollout_fwd(vtp_ctx *ctx, uint32_t nsteps, uint32_
                 cp->x.alloc(c.n);
                 cp->v.alloc(c.n);
                 cp->orig.alloc(c.n);
+                cp->at_cap.alloc(1);
                 A.ckpt.push_back(std::move(cp));
             }
         }

@@ -72,7 +72,11 @@ __global__ void k_reorder_state(const uint32_t *__restrict__ perm, const uint32_
 }
 
 __global__ void k_step_begin(uint32_t *status) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) status[kStDead] = 0;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        status[kStDead] = 0;
+        status[kStNbrPrev] = status[kStNbrAtCap];
+        status[kStNbrAtCap] = 0;
+    }
 }
 
 void common_reorder(Ctx &c, const float4 *xs, float4 *xd, float4 *xd2, const float4 *vs, float4 *vd,
@@ -122,6 +126,7 @@ void common_upload_and_search(Ctx &c) {
     VTP_HIP(hipMemcpyAsync(c.scene.p, c.scene_attach.p, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c.stream));
     hipLaunchKernelGGL(k_iota, dim3(div_up(n, 256)), dim3(256), 0, c.stream, c.orig.p, n);
     c.ugns.allocate(n, c.grid, c.p.max_neighbor_num);
+    c.ugns.status = c.status.p;
     launch_step_begin(c);
     launch_hash(c, c.X[1].p);
     c.ugns.sort_and_rank(c.orig.p, c.stream, c.timer);
@@ -620,6 +625,11 @@ extern "C" vtp_status vtp_set_option(vtp_ctx *ctx, const char *key, double value
         } else if (k == "lanes") {
             if (value != 1.0 && value != 4.0) throw std::runtime_error("lanes must be 1 or 4");
             c.opt_lanes = (uint32_t)value;
+            pbf_capture_reset(c);
+        } else if (k == "neighbor_walk") {
+            if (value != 0.0 && value != 1.0 && value != 2.0)
+                throw std::runtime_error("neighbor_walk must be 0 (adaptive), 1 (row) or 2 (reference)");
+            c.ugns.walk = (uint32_t)value;
             pbf_capture_reset(c);
         } else {
             throw std::runtime_error("unknown option '" + k + "'");

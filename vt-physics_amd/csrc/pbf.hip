@@ -67,7 +67,7 @@ __global__ void k_pbf_predict_hash(float4 *__restrict__ x, float4 *__restrict__ 
 //   lambda_i = -C_i / (S_i + 1e-6)          (mass-free gradient, as the reference).
 // Boundary lambda is defined as 0 (the reference never writes it).
 template <bool MW, uint32_t L>
-__global__ __launch_bounds__(256) void k_pbf_lambda(const float4 *__restrict__ x,
+__global__ __launch_bounds__(256, VTP_PASS_MIN_WAVES) void k_pbf_lambda(const float4 *__restrict__ x,
                                                     const int32_t *__restrict__ mat,
                                                     const uint32_t *__restrict__ nbr,
                                                     const uint32_t *__restrict__ cnt, uint32_t n,
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void k_pbf_lambda(const float4 *__restrict__ x
 // 16-byte gather per neighbour.  MW (extension) weights the pair by m_j, read
 // from the position buffer.
 template <bool MW, uint32_t L>
-__global__ __launch_bounds__(256) void k_pbf_dx(const float4 *__restrict__ x, const float4 *__restrict__ xl,
+__global__ __launch_bounds__(256, VTP_PASS_MIN_WAVES) void k_pbf_dx(const float4 *__restrict__ x, const float4 *__restrict__ xl,
                                                 float4 *__restrict__ xo, float4 *__restrict__ v,
                                                 const int32_t *__restrict__ mat, const uint32_t *__restrict__ nbr,
                                                 const uint32_t *__restrict__ cnt, uint32_t n, uint32_t stride,
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void k_pbf_dx(const float4 *__restrict__ x, co
 // Writes every slot of the output buffers (boundary slots are copied), so the
 // state buffers X[0]/V[0] are complete after the pass.
 template <uint32_t L>
-__global__ __launch_bounds__(256) void k_pbf_xsph(const float4 *__restrict__ x, float4 *__restrict__ xo,
+__global__ __launch_bounds__(256, VTP_PASS_MIN_WAVES) void k_pbf_xsph(const float4 *__restrict__ x, float4 *__restrict__ xo,
                                                   const float4 *__restrict__ v, float4 *__restrict__ vo,
                                                   const int32_t *__restrict__ mat,
                                                   const uint32_t *__restrict__ nbr,

@@ -10,8 +10,11 @@
 //   IMM-CT        + Q[0]
 // Everything else a tick reads is rebuilt at its start (hash, sort, lists,
 // per-slot material/scene/object tables) or derived from the static
-// attach-order tables.  Phase-major arrays are stored packed ([k][n]) rather
-// than with the device stride.
+// attach-order tables, except the count of particles that reached the
+// neighbour cap in the last build: it selects the walk of the next build,
+// which fixes the order inside the lists, so it travels in the header.
+// Phase-major arrays are stored packed ([k][n]) rather than with the device
+// stride.
 #include <cstring>
 #include <vector>
 
@@ -150,6 +153,7 @@ extern "C" vtp_status vtp_save_state(vtp_ctx *ctx, void *host_dst, size_t bytes)
         h.total_incomp_iterations = c.total_inc;
         h.iter_hint_div = c.iter_hint_div;
         h.iter_hint_incomp = c.iter_hint_inc;
+        h.nbr_at_cap = c.h_status[kStNbrAtCap];
         h.payload_bytes = need - sizeof h;
         unsigned char *out = static_cast<unsigned char *>(host_dst);
         std::memcpy(out, &h, sizeof h);
@@ -209,6 +213,7 @@ extern "C" vtp_status vtp_load_state(vtp_ctx *ctx, const void *host_src, size_t 
         // sticky crash bits travel with the state; the per-tick words restart
         uint32_t words[kStWords] = {0};
         words[kStCrash] = h.crash_flags;
+        words[kStNbrAtCap] = h.nbr_at_cap;
         VTP_HIP(hipMemcpyAsync(c.status.p, words, sizeof words, hipMemcpyHostToDevice, c.stream));
         VTP_HIP(hipStreamSynchronize(c.stream));
         c.steps = h.steps;

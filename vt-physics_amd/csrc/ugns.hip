@@ -29,7 +29,22 @@ constexpr int kRsTile = kRsThreads * kRsRounds; // 1024 keys per workgroup (1.3k
 // Neighbour build: 128 threads, each with a private LDS column of up to
 // kNbrStage words.
 constexpr uint32_t kNbrBuildThreads = 128;
-constexpr uint32_t kNbrStage = 32;   // list entries per thread staged in LDS (16 KB per workgroup)
+#ifndef VTP_NBR_STAGE
+#define VTP_NBR_STAGE 32
+#endif
+#ifndef VTP_BUILD_BATCH
+#define VTP_BUILD_BATCH 4
+#endif
+#ifndef VTP_REF_BATCH
+#define VTP_REF_BATCH 8
+#endif
+#ifndef VTP_REF_ORDER_SHARE
+#define VTP_REF_ORDER_SHARE 32
+#endif
+constexpr uint32_t kNbrStage = VTP_NBR_STAGE;   // list entries per thread staged in LDS (16 KB per workgroup)
+// A build walks every particle in the reference's cell order when more than
+// 1/kRefOrderShare of the particles reached the cap in the previous build.
+constexpr uint32_t kRefOrderShare = VTP_REF_ORDER_SHARE;
 
 __global__ void k_fill_u32(uint32_t *a, uint32_t n, uint32_t v) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -212,55 +227,75 @@ __global__ void k_cell_fixup(const uint32_t *__restrict__ key, const uint32_t *_
 // cells in the reference's offset order (x outermost, z innermost,
 // UniformGridNeighborSearch.cpp:33-60), every other particle of a cell in
 // slot order that passes 0 < |x_i - x_j| <= h, capped at `cap` entries.
+// Nothing after the cap-th hit can change the list, so the walk stops once
+// the cap is reached (checked between batches of candidates; hits past the
+// cap inside a batch are counted but not stored) and returns min(hits, cap).
 template <typename Put>
 __device__ uint32_t reference_scan(const float4 *__restrict__ x, const uint32_t *__restrict__ start,
-                                   const uint32_t *__restrict__ end, const Grid &g, uint32_t i, float3 p, int cx,
-                                   int cy, int cz, uint32_t sbase, uint32_t cap, Put &&put) {
+                                   const uint32_t *__restrict__ end, const Grid &g, float3 p, int cx, int cy,
+                                   int cz, uint32_t sbase, uint32_t cap, Put &&put) {
     const uint32_t gxy = (uint32_t)(g.gx * g.gy);
     uint32_t c = 0;
+    // j == i gives d2 == 0, which the test rejects (the reference skips it)
+    auto test = [&](uint32_t j, const float4 &xj) {
+        if (is_neighbor(dist2_exact(p, xyz(xj)), g.nbr_d2_max)) {
+            if (c < cap) put(c, j);
+            ++c;
+        }
+    };
+    constexpr uint32_t B = VTP_REF_BATCH;
     for (int ox = -1; ox <= 1; ++ox) {
-        int X = cx + ox;
+        const int X = cx + ox;
         if (X < 0 || X >= g.gx) continue;
         for (int oy = -1; oy <= 1; ++oy) {
-            int Y = cy + oy;
+            const int Y = cy + oy;
             if (Y < 0 || Y >= g.gy) continue;
             for (int oz = -1; oz <= 1; ++oz) {
-                int Z = cz + oz;
+                const int Z = cz + oz;
                 if (Z < 0 || Z >= g.gz) continue;
-                uint32_t cid = sbase + (uint32_t)Z * gxy + (uint32_t)Y * (uint32_t)g.gx + (uint32_t)X;
-                uint32_t s = start[cid];
-                if (s == kEmpty) continue;
-                uint32_t e = end[cid];
-                for (uint32_t j = s; j <= e; ++j) {
-                    if (j == i) continue;
-                    float d2 = dist2_exact(p, xyz(x[j]));
-                    if (is_neighbor(d2, g.nbr_d2_max) && c < cap) {
-                        put(c, j);
-                        ++c;
-                    }
+                const uint32_t cid = sbase + (uint32_t)Z * gxy + (uint32_t)Y * (uint32_t)g.gx + (uint32_t)X;
+                uint32_t j = ldg(start, cid);
+                if (j == kEmpty) continue;
+                const uint32_t e = ldg(end, cid);
+                // batches of B; the last one loads slot e again in place of
+                // slots past the cell and skips their tests
+                for (; j <= e && c < cap; j += B) {
+                    float4 a[B];
+#pragma unroll
+                    for (uint32_t b = 0; b < B; ++b) a[b] = ldg(x, min(j + b, e));
+#pragma unroll
+                    for (uint32_t b = 0; b < B; ++b)
+                        if (j + b <= e) test(j + b, a[b]);
                 }
+                if (c >= cap) return cap;
             }
         }
     }
     return c;
 }
 
-// Neighbour lists.  Cells (X-1, X, X+1) of one (Y, Z) row have consecutive
-// keys, so their particles occupy one contiguous slot range: the 27 cells
-// are walked as 9 ranges with batched candidate loads.  The resulting list
-// holds the same set as the reference traversal whenever nothing is cut by
-// the cap; a particle with more than `cap` candidates that pass is rebuilt
-// with the reference traversal so the truncated set is the reference's.
-// (The order inside a list only matters for truncation; the reference-layout
-// export re-sorts every list into the reference order.)
+// Neighbour lists.  Cells of one (Y, Z) row have consecutive keys, so the
+// particles of adjacent cells of a row occupy one contiguous slot range.  The
+// reference visits the 27 cells with the x offset outermost (x-1 cells, then
+// x, then x+1; y and z inside) and keeps the first `cap` neighbours it finds.
+// Two walks give exactly that list:
+//   row walk (default): two sweeps over the 9 rows, first cells (X-1, X) of
+//     each row as one contiguous range, then cell X+1 of each row in the
+//     reference's row order.  While the first sweep finds at most `cap`
+//     neighbours the reference keeps all of them, and inside the second the
+//     order is the reference's (rows in (y, z) order, slot order inside a
+//     cell), so cutting at `cap` keeps the reference's set.  A particle whose
+//     x-1 and x cells alone hold more than `cap` neighbours is rebuilt with
+//     the reference traversal.
+//   reference walk: the reference traversal for every particle, stopping at
+//     the cap.  27 cell walks instead of 18 ranges, but no rebuilds.
+// The build counts the particles that reach the cap into kStNbrAtCap;
+// k_step_begin moves the count to kStNbrPrev, and the next build takes the
+// reference walk when more than 1/kRefOrderShare of the particles reached
+// the cap (the choice is read from device memory, so a captured graph adapts
+// as well).  Both walks give the same lists; the order inside a list is free
+// (the reference-layout export re-sorts every list into the reference order).
 //
-// kStaged: each thread assembles the first kNbrStage entries of its list in
-// a private LDS column (s_list[c * blockDim + tid]) and the wave then writes
-// those ELL columns out one at a time, so every store of the wave covers 64
-// consecutive slots of one column.  Written straight to HBM, the lanes'
-// appends land in `cap` different columns at once and the L2 evicts most
-// lines half-written.  Entries past kNbrStage (dense regions only) are
-// written directly; the smaller LDS footprint keeps occupancy up.
 template <bool kStaged>
 __global__ __launch_bounds__(kNbrBuildThreads) void k_build_neighbors(const float4 *__restrict__ x,
                                                                       const uint32_t *__restrict__ key,
@@ -268,15 +303,20 @@ __global__ __launch_bounds__(kNbrBuildThreads) void k_build_neighbors(const floa
                                                                       const uint32_t *__restrict__ end, Grid g,
                                                                       uint32_t n, uint32_t stride, uint32_t cap,
                                                                       uint32_t *__restrict__ nbr,
-                                                                      uint32_t *__restrict__ cnt) {
+                                                                      uint32_t *__restrict__ cnt,
+                                                                      uint32_t *__restrict__ status,
+                                                                      uint32_t walk) {
     extern __shared__ uint32_t s_list[];
     const uint32_t i = slot_index();
+    // uniform over the grid; walk = 1 / 2 forces the row / reference walk
+    const bool ref_walk = walk == 2 || (walk == 0 && status[kStNbrPrev] > n / kRefOrderShare);
     const uint32_t stage = kStaged ? min(cap, kNbrStage) : 0u;
     auto put = [&](uint32_t c, uint32_t j) {
         if (c < stage) s_list[c * blockDim.x + threadIdx.x] = j;
         else nbr[(size_t)c * stride + i] = j;
     };
     uint32_t c = 0;
+    bool full = false;
     if (i < n) {
         const uint32_t k = key[i];
         if (k >= g.dead_key) {
@@ -291,45 +331,72 @@ __global__ __launch_bounds__(kNbrBuildThreads) void k_build_neighbors(const floa
             const int cx = (int)(rem - (uint32_t)cy * (uint32_t)g.gx);
             const uint32_t sbase = scene * g.cells;
             const float3 p = xyz(x[i]);
-            const int X0 = max(cx - 1, 0), X1 = min(cx + 1, g.gx - 1);
             auto test = [&](uint32_t j, const float4 &xj) {
-                float d2 = dist2_exact(p, xyz(xj));
-                if (j != i && is_neighbor(d2, g.nbr_d2_max)) {
+                // j == i gives d2 == 0, which the test rejects
+                if (is_neighbor(dist2_exact(p, xyz(xj)), g.nbr_d2_max)) {
                     if (c < cap) put(c, j);
                     ++c;
                 }
             };
-            for (int oy = -1; oy <= 1; ++oy) {
-                const int Y = cy + oy;
-                if (Y < 0 || Y >= g.gy) continue;
-                for (int oz = -1; oz <= 1; ++oz) {
-                    const int Z = cz + oz;
-                    if (Z < 0 || Z >= g.gz) continue;
-                    const uint32_t row = sbase + (uint32_t)Z * gxy + (uint32_t)Y * (uint32_t)g.gx;
-                    uint32_t s = kEmpty, e = 0;
-                    for (int X = X0; X <= X1; ++X) {
-                        uint32_t a = ldg(start, row + X);
-                        if (a != kEmpty) {
-                            if (s == kEmpty) s = a;
-                            e = ldg(end, row + X);
+            auto walk = [&](uint32_t j, uint32_t e) {
+                constexpr uint32_t B = VTP_BUILD_BATCH;   // candidates in flight
+                for (; j <= e; j += B) {
+                    float4 a[B];
+#pragma unroll
+                    for (uint32_t b = 0; b < B; ++b) a[b] = ldg(x, min(j + b, e));
+#pragma unroll
+                    for (uint32_t b = 0; b < B; ++b)
+                        if (j + b <= e) test(j + b, a[b]);
+                }
+            };
+            if (ref_walk) {
+                c = reference_scan(x, start, end, g, p, cx, cy, cz, sbase, cap, put);
+            } else {
+                const int XA = max(cx - 1, 0);
+                // sweep 1: cells (X-1, X) of the 9 rows
+                for (int oy = -1; oy <= 1; ++oy) {
+                    const int Y = cy + oy;
+                    if (Y < 0 || Y >= g.gy) continue;
+                    for (int oz = -1; oz <= 1; ++oz) {
+                        const int Z = cz + oz;
+                        if (Z < 0 || Z >= g.gz) continue;
+                        const uint32_t row = sbase + (uint32_t)Z * gxy + (uint32_t)Y * (uint32_t)g.gx;
+                        uint32_t s = ldg(start, row + XA), e = ldg(end, row + XA);
+                        if (XA != cx) {
+                            const uint32_t s1 = ldg(start, row + cx);
+                            if (s1 != kEmpty) {
+                                if (s == kEmpty) s = s1;
+                                e = ldg(end, row + cx);
+                            }
+                        }
+                        if (s != kEmpty && c <= cap) walk(s, e);   // past the cap: rebuilt below
+                    }
+                }
+                const uint32_t c_front = c;
+                // sweep 2: cell X+1 of the 9 rows, in the reference's order
+                if (cx + 1 < g.gx && c_front <= cap) {
+                    for (int oy = -1; oy <= 1; ++oy) {
+                        const int Y = cy + oy;
+                        if (Y < 0 || Y >= g.gy) continue;
+                        for (int oz = -1; oz <= 1; ++oz) {
+                            const int Z = cz + oz;
+                            if (Z < 0 || Z >= g.gz) continue;
+                            const uint32_t cell = sbase + (uint32_t)Z * gxy + (uint32_t)Y * (uint32_t)g.gx + cx + 1;
+                            const uint32_t s = ldg(start, cell);
+                            if (s != kEmpty) walk(s, ldg(end, cell));
                         }
                     }
-                    if (s == kEmpty) continue;
-                    uint32_t j = s;
-                    for (; j + 4 <= e + 1; j += 4) {
-                        float4 a0 = ldg(x, j), a1 = ldg(x, j + 1), a2 = ldg(x, j + 2), a3 = ldg(x, j + 3);
-                        test(j, a0);
-                        test(j + 1, a1);
-                        test(j + 2, a2);
-                        test(j + 3, a3);
-                    }
-                    for (; j <= e; ++j) test(j, ldg(x, j));
                 }
+                if (c_front > cap) c = reference_scan(x, start, end, g, p, cx, cy, cz, sbase, cap, put);
+                else c = min(c, cap);
             }
-            if (c > cap) c = reference_scan(x, start, end, g, i, p, cx, cy, cz, sbase, cap, put);
             cnt[i] = c;
+            full = cap > 0 && c == cap;
         }
     }
+    // particles that reached the cap steer the walk of the next build
+    const uint64_t m = __ballot(full);
+    if (m && (threadIdx.x & 63u) == 0) atomicAdd(&status[kStNbrAtCap], (uint32_t)__popcll(m));
     if (kStaged) {
         // every lane reads back only its own LDS column: no barrier needed
         const uint32_t cmax = min(wave_max_u32(c), stage);
@@ -447,10 +514,11 @@ void Ugns::sort_and_rank(const uint32_t *orig_old, hipStream_t s, StageTimer &tm
 }
 
 void Ugns::build_neighbors(const float4 *x, hipStream_t s, StageTimer &tm) {
+    if (!status) throw std::runtime_error("neighbour build without status words");
     tm.begin(s, "ugns_neighbors");
     const size_t lds = (size_t)std::min(cap, kNbrStage) * kNbrBuildThreads * sizeof(uint32_t);
     hipLaunchKernelGGL(k_build_neighbors<true>, dim3(blocks_for(n, kNbrBuildThreads)), dim3(kNbrBuildThreads), lds,
-                       s, x, key[sorted].p, start.p, end.p, g, n, stride, cap, nbr.p, cnt.p);
+                       s, x, key[sorted].p, start.p, end.p, g, n, stride, cap, nbr.p, cnt.p, status, walk);
     tm.end(s);
 }
 

@@ -7,6 +7,19 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Build-time tuning of the neighbour loops (values measured on MI355X at
+// PBF/DFSPH 1M particles):
+//   VTP_NBR_BATCH      neighbours in flight per lane when a neighbour's data
+//                      is at most 32 bytes (larger payloads use half)
+//   VTP_PASS_MIN_WAVES minimum waves per SIMD the PBF neighbour passes are
+//                      compiled for (bounds their registers)
+#ifndef VTP_NBR_BATCH
+#define VTP_NBR_BATCH 8
+#endif
+#ifndef VTP_PASS_MIN_WAVES
+#define VTP_PASS_MIN_WAVES 1
+#endif
+
 namespace vtp {
 
 // Launch-side description of the uniform grid, passed by value.
@@ -188,7 +201,7 @@ __device__ __forceinline__ void for_each_neighbor(const uint32_t *__restrict__ n
     const uint32_t off = i * (uint32_t)sizeof(uint32_t);
     auto idx = [&](uint32_t t) { return *reinterpret_cast<const uint32_t *>(base + t * step + off); };
     using D = decltype(load(0u));
-    constexpr uint32_t kNbrBatch = sizeof(D) > 32 ? 4 : 8;
+    constexpr uint32_t kNbrBatch = sizeof(D) > 32 ? VTP_NBR_BATCH / 2 : VTP_NBR_BATCH;
     uint32_t t = 0;
     if (nn >= kNbrBatch) {
         uint32_t j[kNbrBatch];

@@ -115,6 +115,8 @@ struct Ugns {
     DevBuf<uint32_t> start, end;    // [dead_key + 1], inclusive end, kEmpty = empty
     DevBuf<uint32_t> so, perm, orig_new;
     DevBuf<uint32_t> nbr, cnt;      // ELL lists nbr[t*stride + slot], count word per slot
+    uint32_t *status = nullptr;     // the context's status words (cap counts)
+    uint32_t walk = 0;              // list walk: 0 adaptive, 1 row walk, 2 reference walk
 
     void allocate(uint32_t n, const Grid &g, uint32_t max_neighbors);
     void sort_and_rank(const uint32_t *orig_old, hipStream_t s, StageTimer &tm);
@@ -135,6 +137,8 @@ enum StatusWord : uint32_t {
     kStDone = 5,         // solver-loop convergence flag
     kStIter = 6,         // running iteration counter of the active loop
     kStNegCount = 7,     // newly flagged negative fractions (IMM)
+    kStNbrAtCap = 8,     // particles whose list reached the cap in the running build
+    kStNbrPrev = 9,      // the same count of the previous build (steers the walk)
     kStWords = 16
 };
 

@@ -117,6 +117,8 @@ class VtpStateHeader(C.Structure):
         ("total_incomp_iterations", C.c_uint64),
         ("iter_hint_div", C.c_uint32),
         ("iter_hint_incomp", C.c_uint32),
+        ("nbr_at_cap", C.c_uint32),
+        ("reserved", C.c_uint32),
         ("payload_bytes", C.c_uint64),
     ]
 
