@@ -110,9 +110,13 @@ __host__ inline Cubic make_cubic(float h) {
 // Squared-distance bound of the reference's q < 1e-6 gradient cut-off.
 __device__ __forceinline__ float cubic_r2_lo(const Cubic &k) { return (1e-6f * k.h) * (1e-6f * k.h); }
 
+// rsqrt of max(r2, FLT_MIN): finite for a coincident pair, whose q is then 0
+// (W = sigma), so no neighbour loop branches on r2 == 0.  Identical to
+// rsqrt(r2) for every normal r2.
+__device__ __forceinline__ float cubic_rinv(float r2) { return __frsqrt_rn(fmaxf(r2, 1.17549435e-38f)); }
+
 __device__ __forceinline__ float cubic_w(float r2, const Cubic &k) {
-    const float r = r2 > 0.0f ? r2 * __frsqrt_rn(r2) : 0.0f;
-    const float q = r * k.inv_h;
+    const float q = (r2 * cubic_rinv(r2)) * k.inv_h;
     const float t = fmaxf(1.0f - q, 0.0f);
     const float wi = fmaf(q * q, fmaf(q, 6.0f, -6.0f), 1.0f);
     const float wo = 2.0f * (t * t) * t;
@@ -121,18 +125,18 @@ __device__ __forceinline__ float cubic_w(float r2, const Cubic &k) {
 
 // Scalar factor f such that gradW(r_vec) = f * r_vec (zero for q < 1e-6).
 __device__ __forceinline__ float cubic_grad_factor(float r2, const Cubic &k) {
-    const float rinv = __frsqrt_rn(r2);
+    const float rinv = cubic_rinv(r2);
     const float q = (r2 * rinv) * k.inv_h;
     const float t = fmaxf(1.0f - q, 0.0f);
     const float gi = q * fmaf(q, 18.0f, -12.0f);
     const float go = -6.0f * (t * t);
     const float f = k.grad_scale * (q <= 0.5f ? gi : go) * rinv;
-    return r2 >= cubic_r2_lo(k) ? f : 0.0f;   // also r2 == 0 (rinv = inf)
+    return r2 >= cubic_r2_lo(k) ? f : 0.0f;
 }
 
-// W and the gradient factor together (one rsqrt).
+// W and the gradient factor together (one rsqrt), straight-line code.
 __device__ __forceinline__ void cubic_w_grad(float r2, const Cubic &k, float &w, float &f) {
-    const float rinv = __frsqrt_rn(r2);
+    const float rinv = cubic_rinv(r2);
     const float q = (r2 * rinv) * k.inv_h;
     const float t = fmaxf(1.0f - q, 0.0f);
     const float t2 = t * t;
@@ -141,10 +145,9 @@ __device__ __forceinline__ void cubic_w_grad(float r2, const Cubic &k, float &w,
     const float wo = 2.0f * t2 * t;
     const float gi = q * fmaf(q, 18.0f, -12.0f);
     const float go = -6.0f * t2;
-    const bool ok = r2 >= cubic_r2_lo(k);
-    // r2 == 0 only for a coincident pair: W(0) = sigma, gradient 0
-    w = ok ? k.sigma * (inner ? wi : wo) : (r2 > 0.0f ? k.sigma * wi : k.sigma);
-    f = ok ? k.grad_scale * (inner ? gi : go) * rinv : 0.0f;
+    w = k.sigma * (inner ? wi : wo);   // r2 == 0: q = 0, W = sigma
+    const float g = k.grad_scale * (inner ? gi : go) * rinv;
+    f = r2 >= cubic_r2_lo(k) ? g : 0.0f;
 }
 
 // Akinci cohesion spline C(r) (KernelFunctions.cuh:58-69), c = 32/(pi h^9).
