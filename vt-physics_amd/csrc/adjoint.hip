@@ -50,8 +50,16 @@ __global__ void k_adj_mat_from_orig(const uint32_t *__restrict__ orig, const int
 }
 
 // u = xbar dt + vbar (adjoint of dv), zero for non-fluid.
+// The reverse passes gather a neighbour's data as one 32- or 64-byte record
+// (one cache line) instead of one load from each of several arrays; the
+// record of slot i is written by the per-particle kernel that produces its
+// last field.
+//   xvu[4i .. 4i+2] = (x_i, v_i, u_i)           for the XSPH adjoint
+//   xw [2i .. 2i+1] = (x_i, w_i)                for lambda_bar
+//   rec[4i .. 4i+3] = (x_i, (w_i, l_i), (G_i, rb_i), (sb_i, 0, 0, 0))  for xbar
 __global__ void k_adj_u(const float4 *__restrict__ xb, const float4 *__restrict__ vb,
-                        const int32_t *__restrict__ mat, uint32_t n, float dt, float4 *__restrict__ u) {
+                        const int32_t *__restrict__ mat, uint32_t n, float dt, const float4 *__restrict__ X,
+                        const float4 *__restrict__ V, float4 *__restrict__ xvu) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 a = make_float4(0, 0, 0, 0);
@@ -59,12 +67,15 @@ __global__ void k_adj_u(const float4 *__restrict__ xb, const float4 *__restrict_
         float4 x = xb[i], v = vb[i];
         a = make_float4(x.x * dt + v.x, x.y * dt + v.y, x.z * dt + v.z, 0.0f);
     }
-    u[i] = a;
+    xvu[4 * (size_t)i] = X[i];
+    xvu[4 * (size_t)i + 1] = V[i];
+    xvu[4 * (size_t)i + 2] = a;
 }
 
 // w = xbar + vbar/dt (adjoint of dx of a constraint iteration), zero for non-fluid.
 __global__ void k_adj_w(const float4 *__restrict__ xb, const float4 *__restrict__ vb,
-                        const int32_t *__restrict__ mat, uint32_t n, float inv_dt, float4 *__restrict__ w) {
+                        const int32_t *__restrict__ mat, uint32_t n, float inv_dt, const float4 *__restrict__ X,
+                        float4 *__restrict__ xw) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 a = make_float4(0, 0, 0, 0);
@@ -72,28 +83,34 @@ __global__ void k_adj_w(const float4 *__restrict__ xb, const float4 *__restrict_
         float4 x = xb[i], v = vb[i];
         a = make_float4(x.x + v.x * inv_dt, x.y + v.y * inv_dt, x.z + v.z * inv_dt, 0.0f);
     }
-    w[i] = a;
+    xw[2 * (size_t)i] = X[i];
+    xw[2 * (size_t)i + 1] = a;
 }
 
 // Transposed XSPH Jacobian:
 //   xbar_p += k sum_j [V_j u_p.(v_j - v_p) + V_p u_j.(v_p - v_j)] gradW_pj
 //   vbar_p += k V_p sum_j W_pj u_j - k u_p sum_j V_j W_pj
 //   kbar   += sum_p u_p . sum_j V_j (v_j - v_p) W_pj
-__global__ __launch_bounds__(256) void k_adj_xsph(const float4 *__restrict__ X, const float4 *__restrict__ V,
-                                                  const float4 *__restrict__ U, const int32_t *__restrict__ mat,
+// L lanes per particle (the list is split over them and the sums combined):
+// the reverse passes run on small scenes, where one lane per particle leaves
+// most of the GPU idle behind each particle's chain of dependent gathers.
+template <uint32_t L>
+__global__ __launch_bounds__(256) void k_adj_xsph(const float4 *__restrict__ xvu, const int32_t *__restrict__ mat,
                                                   const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ cnt,
                                                   uint32_t n, uint32_t stride, PbfK q, float4 *__restrict__ xb,
                                                   float4 *__restrict__ vb, double *kbar) {
-    uint32_t p = slot_index();
+    const uint32_t g = slot_index(), p = g / L, sub = g % L;
+    const bool fluid = p < n && mat[p] == VTP_MAT_FLUID;
     double kp = 0.0;
-    if (p < n && mat[p] == VTP_MAT_FLUID) {
-        const float4 xp = X[p], vp = V[p], up = U[p];
+    if (fluid) {
+        const float4 xp = xvu[4 * (size_t)p], vp = xvu[4 * (size_t)p + 1], up = xvu[4 * (size_t)p + 2];
         const float k = q.xsph_k;
         const uint32_t nn = cnt[p] & ~kDeadBit;
         float ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0, sw = 0, kk = 0;
         struct D { float4 x, v, u; };
-        for_each_neighbor(
-            nbr, stride, p, nn, [&](uint32_t j) { return D{ldg(X, j), ldg(V, j), ldg(U, j)}; },
+        for_each_neighbor_coop<L>(
+            nbr, stride, p, nn, sub,
+            [&](uint32_t j) { return D{ldg(xvu, 4 * j), ldg(xvu, 4 * j + 1), ldg(xvu, 4 * j + 2)}; },
             [&](const D &e) {
                 float dx = xp.x - e.x.x, dy = xp.y - e.x.y, dz = xp.z - e.x.z;
                 float W, f;
@@ -112,16 +129,26 @@ __global__ __launch_bounds__(256) void k_adj_xsph(const float4 *__restrict__ X, 
                 sw = fmaf(e.v.w, W, sw);
                 kk = fmaf(e.v.w * W, upd, kk);
             });
-        float4 a = xb[p], b = vb[p];
-        a.x += ax;
-        a.y += ay;
-        a.z += az;
-        b.x += bx - k * sw * up.x;
-        b.y += by - k * sw * up.y;
-        b.z += bz - k * sw * up.z;
-        xb[p] = a;
-        vb[p] = b;
-        kp = kk;
+        ax = lane_sum<L>(ax);
+        ay = lane_sum<L>(ay);
+        az = lane_sum<L>(az);
+        bx = lane_sum<L>(bx);
+        by = lane_sum<L>(by);
+        bz = lane_sum<L>(bz);
+        sw = lane_sum<L>(sw);
+        kk = lane_sum<L>(kk);
+        if (sub == 0) {
+            float4 a = xb[p], b = vb[p];
+            a.x += ax;
+            a.y += ay;
+            a.z += az;
+            b.x += bx - k * sw * up.x;
+            b.y += by - k * sw * up.y;
+            b.z += bz - k * sw * up.z;
+            xb[p] = a;
+            vb[p] = b;
+            kp = kk;
+        }
     }
     double s = wave_sum(kp);
     if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(kbar, s);
@@ -129,57 +156,74 @@ __global__ __launch_bounds__(256) void k_adj_xsph(const float4 *__restrict__ X, 
 
 // lambda_bar_i = (1/rho0) sum_j (w_i - w_j) . gradW_ij, then the adjoints of the
 // density (rb, zero where clamped) and of S (sb): rs = (rb, sb).
-__global__ __launch_bounds__(256) void k_adj_lambar(const float4 *__restrict__ X, const float4 *__restrict__ Wv,
-                                                    const float *__restrict__ Cc, const float4 *__restrict__ sg,
+template <uint32_t L>
+__global__ __launch_bounds__(256) void k_adj_lambar(const float4 *__restrict__ xw, const float *__restrict__ Cc,
+                                                    const float4 *__restrict__ sg, const float *__restrict__ lam,
                                                     const int32_t *__restrict__ mat, const uint32_t *__restrict__ nbr,
                                                     const uint32_t *__restrict__ cnt, uint32_t n, uint32_t stride,
-                                                    PbfK q, float2 *__restrict__ rs) {
-    uint32_t i = slot_index();
+                                                    PbfK q, float4 *__restrict__ rec) {
+    const uint32_t g = slot_index(), i = g / L, sub = g % L;
     if (i >= n) return;
+    const float4 xi = xw[2 * (size_t)i], wi = xw[2 * (size_t)i + 1], Gi = sg[i];
+    float4 *const r = rec + 4 * (size_t)i;
     if (mat[i] != VTP_MAT_FLUID) {
-        rs[i] = make_float2(0, 0);
+        if (sub == 0) {
+            r[0] = xi;
+            r[1] = make_float4(wi.x, wi.y, wi.z, lam[i]);
+            r[2] = make_float4(Gi.x, Gi.y, Gi.z, 0.0f);
+            r[3] = make_float4(0, 0, 0, 0);
+        }
         return;
     }
-    const float4 xi = X[i], wi = Wv[i];
     const uint32_t nn = cnt[i] & ~kDeadBit;
     float lb = 0.0f;
     struct D { float4 x, w; };
-    for_each_neighbor(
-        nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(X, j), ldg(Wv, j)}; },
+    for_each_neighbor_coop<L>(
+        nbr, stride, i, nn, sub, [&](uint32_t j) { return D{ldg(xw, 2 * j), ldg(xw, 2 * j + 1)}; },
         [&](const D &e) {
             float dx = xi.x - e.x.x, dy = xi.y - e.x.y, dz = xi.z - e.x.z;
             float f = cubic_grad_factor(dx * dx + dy * dy + dz * dz, q.k);
             lb = fmaf(((wi.x - e.w.x) * dx + (wi.y - e.w.y) * dy + (wi.z - e.w.z) * dz), f, lb);
         });
-    lb *= q.inv_rho0;
+    lb = lane_sum<L>(lb) * q.inv_rho0;
+    if (sub != 0) return;
+    r[0] = xi;
+    r[1] = make_float4(wi.x, wi.y, wi.z, lam[i]);
     const float C = Cc[i];
     const float den = sg[i].w + 1e-6f;
     const float Cb = -lb / den;
     const float Sb = lb * C / (den * den);
     const float rb = C > 0.0f ? Cb * q.inv_rho0 : 0.0f;
-    rs[i] = make_float2(rb, Sb);
+    r[2] = make_float4(Gi.x, Gi.y, Gi.z, rb);
+    r[3] = make_float4(Sb, 0, 0, 0);
 }
 
 // Transposed constraint-iteration Jacobian:
 //   xbar_p += sum_j H_pj [ (l_p + l_j)/rho0 (w_p - w_j)
 //                          + 2/rho0^2 (sb_p (gradW_pj + G_p) + sb_j (gradW_pj - G_j)) ]
 //            + gradW_pj (rb_p m_j + rb_j m_p)
-__global__ __launch_bounds__(256) void k_adj_xbar(const float4 *__restrict__ X, const float4 *__restrict__ Wv,
-                                                  const float2 *__restrict__ rs, const float *__restrict__ lam,
-                                                  const float4 *__restrict__ sg, const int32_t *__restrict__ mat,
+template <uint32_t L>
+__global__ __launch_bounds__(256) void k_adj_xbar(const float4 *__restrict__ rec, const int32_t *__restrict__ mat,
                                                   const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ cnt,
                                                   uint32_t n, uint32_t stride, PbfK q, float4 *__restrict__ xb) {
-    uint32_t p = slot_index();
+    const uint32_t g = slot_index(), p = g / L, sub = g % L;
     if (p >= n || mat[p] != VTP_MAT_FLUID) return;
-    const float4 xp = X[p], wp = Wv[p], Gp = sg[p];
-    const float2 rp = rs[p];
-    const float lp = lam[p];
+    const float4 *const own = rec + 4 * (size_t)p;
+    const float4 xp = own[0], wl = own[1], gr = own[2];
+    const float4 wp = wl, Gp = gr;
+    const float2 rp = make_float2(gr.w, own[3].x);
+    const float lp = wl.w;
     const float c2 = 2.0f * q.inv_rho0 * q.inv_rho0;
     const uint32_t nn = cnt[p] & ~kDeadBit;
     float ax = 0, ay = 0, az = 0;
     struct D { float4 x, w, G; float2 r; float l; };
-    for_each_neighbor(
-        nbr, stride, p, nn, [&](uint32_t j) { return D{ldg(X, j), ldg(Wv, j), ldg(sg, j), ldg(rs, j), ldg(lam, j)}; },
+    const float *const recf = reinterpret_cast<const float *>(rec);
+    for_each_neighbor_coop<L>(
+        nbr, stride, p, nn, sub,
+        [&](uint32_t j) {
+            const float4 a = ldg(rec, 4 * j), b = ldg(rec, 4 * j + 1), g = ldg(rec, 4 * j + 2);
+            return D{a, b, g, make_float2(g.w, ldg(recf, 16 * j + 12)), b.w};   // one 64-byte record
+        },
         [&](const D &e) {
             float rx = xp.x - e.x.x, ry = xp.y - e.x.y, rz = xp.z - e.x.z;
             float f, d;
@@ -195,6 +239,10 @@ __global__ __launch_bounds__(256) void k_adj_xbar(const float4 *__restrict__ X, 
             ay += f * ty + rt * ry + dm * gy;
             az += f * tz + rt * rz + dm * gz;
         });
+    ax = lane_sum<L>(ax);
+    ay = lane_sum<L>(ay);
+    az = lane_sum<L>(az);
+    if (sub != 0) return;
     float4 a = xb[p];
     a.x += ax;
     a.y += ay;
@@ -288,8 +336,20 @@ struct AdjointState {
     uint32_t nsteps = 0, every = 0, nseg = 0;
     std::vector<std::unique_ptr<Checkpoint>> ckpt;
     std::vector<std::unique_ptr<TapeBufs>> tape;
-    DevBuf<float4> xb, vb, xb2, vb2, U, Wv;
-    DevBuf<float2> rs;
+    // One captured graph per checkpoint segment (reload, taped re-forward,
+    // reverse), replayed by every later backward of the same rollout shape.
+    std::vector<hipGraphExec_t> seg_exec;
+    uint64_t seg_epoch = ~0ull;
+    uint32_t seg_nsteps = 0, seg_every = 0;
+    float4 *canon[4] = {};   // xb, vb, xb2, vb2 as allocated (the reverse ping-pongs them)
+    void drop_graphs() {
+        for (auto &e : seg_exec)
+            if (e) (void)hipGraphExecDestroy(e);
+        seg_exec.clear();
+    }
+    ~AdjointState() { drop_graphs(); }
+    DevBuf<float4> xb, vb, xb2, vb2;
+    DevBuf<float4> xvu, xw, rec;   // gather records of the reverse passes
     DevBuf<double> kbar;
     DevBuf<int32_t> mat;
     bool valid = false;
@@ -349,6 +409,10 @@ static void alloc_tape(Ctx &c, TapeBufs &t) {
     t.ptrs.v_last = t.vlast.p;
 }
 
+// Lanes per particle of the reverse passes: 4 on scenes too small to fill the
+// GPU with one lane per particle (measured: 100k particles), else 1.
+static uint32_t adj_lanes(const Ctx &c) { return c.n < kSmallScene ? 4u : 1u; }
+
 static void reverse_tick(Ctx &c, AdjointState &A, TapeBufs &t) {
     const uint32_t n = c.n, K = c.p.iteration_num, nb = div_up(n, 256);
     const PbfK q = pbf_constants(c);
@@ -356,17 +420,31 @@ static void reverse_tick(Ctx &c, AdjointState &A, TapeBufs &t) {
     hipLaunchKernelGGL(k_adj_mat_from_orig, dim3(nb), dim3(256), 0, c.stream, t.orig.p, c.mat_attach.p,
                        c.scene_attach.p, n, A.mat.p, (uint32_t *)nullptr);
     c.timer.begin(c.stream, "adj_xsph");
-    hipLaunchKernelGGL(k_adj_u, dim3(nb), dim3(256), 0, c.stream, A.xb.p, A.vb.p, A.mat.p, n, q.dt, A.U.p);
-    hipLaunchKernelGGL(k_adj_xsph, dim3(nb), dim3(256), 0, c.stream, t.x[K].p, t.vlast.p, A.U.p, A.mat.p, t.nbr.p,
-                       t.cnt.p, n, u.stride, q, A.xb.p, A.vb.p, A.kbar.p);
+    hipLaunchKernelGGL(k_adj_u, dim3(nb), dim3(256), 0, c.stream, A.xb.p, A.vb.p, A.mat.p, n, q.dt, t.x[K].p,
+                       t.vlast.p, A.xvu.p);
+    const uint32_t L = adj_lanes(c), nbl = div_up(n * L, 256);
+    if (L == 4)
+        hipLaunchKernelGGL(k_adj_xsph<4>, dim3(nbl), dim3(256), 0, c.stream, A.xvu.p, A.mat.p, t.nbr.p, t.cnt.p, n,
+                           u.stride, q, A.xb.p, A.vb.p, A.kbar.p);
+    else
+        hipLaunchKernelGGL(k_adj_xsph<1>, dim3(nbl), dim3(256), 0, c.stream, A.xvu.p, A.mat.p, t.nbr.p, t.cnt.p, n,
+                           u.stride, q, A.xb.p, A.vb.p, A.kbar.p);
     c.timer.end(c.stream);
     for (int m = (int)K - 1; m >= 0; --m) {
         c.timer.begin(c.stream, "adj_iteration");
-        hipLaunchKernelGGL(k_adj_w, dim3(nb), dim3(256), 0, c.stream, A.xb.p, A.vb.p, A.mat.p, n, q.inv_dt, A.Wv.p);
-        hipLaunchKernelGGL(k_adj_lambar, dim3(nb), dim3(256), 0, c.stream, t.x[m].p, A.Wv.p, t.C[m].p, t.sg[m].p,
-                           A.mat.p, t.nbr.p, t.cnt.p, n, u.stride, q, A.rs.p);
-        hipLaunchKernelGGL(k_adj_xbar, dim3(nb), dim3(256), 0, c.stream, t.x[m].p, A.Wv.p, A.rs.p, t.lam[m].p,
-                           t.sg[m].p, A.mat.p, t.nbr.p, t.cnt.p, n, u.stride, q, A.xb.p);
+        hipLaunchKernelGGL(k_adj_w, dim3(nb), dim3(256), 0, c.stream, A.xb.p, A.vb.p, A.mat.p, n, q.inv_dt,
+                           t.x[m].p, A.xw.p);
+        if (L == 4) {
+            hipLaunchKernelGGL(k_adj_lambar<4>, dim3(nbl), dim3(256), 0, c.stream, A.xw.p, t.C[m].p, t.sg[m].p,
+                               t.lam[m].p, A.mat.p, t.nbr.p, t.cnt.p, n, u.stride, q, A.rec.p);
+            hipLaunchKernelGGL(k_adj_xbar<4>, dim3(nbl), dim3(256), 0, c.stream, A.rec.p, A.mat.p, t.nbr.p, t.cnt.p,
+                               n, u.stride, q, A.xb.p);
+        } else {
+            hipLaunchKernelGGL(k_adj_lambar<1>, dim3(nbl), dim3(256), 0, c.stream, A.xw.p, t.C[m].p, t.sg[m].p,
+                               t.lam[m].p, A.mat.p, t.nbr.p, t.cnt.p, n, u.stride, q, A.rec.p);
+            hipLaunchKernelGGL(k_adj_xbar<1>, dim3(nbl), dim3(256), 0, c.stream, A.rec.p, A.mat.p, t.nbr.p, t.cnt.p,
+                               n, u.stride, q, A.xb.p);
+        }
         c.timer.end(c.stream);
     }
     hipLaunchKernelGGL(k_adj_unpermute_predict, dim3(nb), dim3(256), 0, c.stream, t.perm.p, A.xb.p, A.vb.p, n, q.dt,
@@ -397,6 +475,8 @@ static vtp_status adj_check(Ctx &c) {
     if (c.p.solver != VTP_PBF) { c.err = "rollout gradients exist for PBF only"; return VTP_EINVAL; }
     if (c.p.mass_weighted) { c.err = "the adjoint differentiates the reference (mass-free) formulation only"; return VTP_EINVAL; }
     if (c.p.iteration_num > 16) { c.err = "the adjoint supports at most 16 constraint iterations"; return VTP_EINVAL; }
+    // gather records are addressed with 32-bit byte offsets (64 bytes per particle)
+    if (c.n > (1u << 26)) { c.err = "the adjoint supports at most 2^26 particles"; return VTP_EINVAL; }
     if (!c.initialized) { c.err = "not initialized"; return VTP_ESTATE; }
     return VTP_OK;
 }
@@ -464,12 +544,29 @@ extern "C" vtp_status vtp_pbf_rollout_bwd(vtp_ctx *ctx, const float *grad_x, con
             A.vb.alloc(n);
             A.xb2.alloc(n);
             A.vb2.alloc(n);
-            A.U.alloc(n);
-            A.Wv.alloc(n);
-            A.rs.alloc(n);
+            A.xvu.alloc(4 * (size_t)n);
+            A.xw.alloc(2 * (size_t)n);
+            A.rec.alloc(4 * (size_t)n);
             A.kbar.alloc(1);
             A.mat.alloc(n);
+            A.canon[0] = A.xb.p;
+            A.canon[1] = A.vb.p;
+            A.canon[2] = A.xb2.p;
+            A.canon[3] = A.vb2.p;
+            A.drop_graphs();   // captured against the old tapes
         }
+        if (A.seg_epoch != c.capture_epoch || A.seg_nsteps != A.nsteps || A.seg_every != A.every ||
+            A.seg_exec.size() != A.nseg) {
+            A.drop_graphs();
+            A.seg_exec.assign(A.nseg, nullptr);
+            A.seg_epoch = c.capture_epoch;
+            A.seg_nsteps = A.nsteps;
+            A.seg_every = A.every;
+        }
+        A.xb.p = A.canon[0];
+        A.vb.p = A.canon[1];
+        A.xb2.p = A.canon[2];
+        A.vb2.p = A.canon[3];
         // terminal adjoints: attach order -> final slot order (non-fluid zero)
         std::vector<uint32_t> orig(n);
         std::vector<int32_t> mat(n);
@@ -487,12 +584,39 @@ extern "C" vtp_status vtp_pbf_rollout_bwd(vtp_ctx *ctx, const float *grad_x, con
         VTP_HIP(hipMemcpyAsync(A.xb.p, hx.data(), n * sizeof(float4), hipMemcpyHostToDevice, c.stream));
         VTP_HIP(hipMemcpyAsync(A.vb.p, hv.data(), n * sizeof(float4), hipMemcpyHostToDevice, c.stream));
         VTP_HIP(hipMemsetAsync(A.kbar.p, 0, sizeof(double), c.stream));
+        const bool graphs = !c.timer.enabled;   // stage timing needs eager launches
         for (int seg = (int)A.nseg - 1; seg >= 0; --seg) {
             const uint32_t s0 = (uint32_t)seg * A.every;
             const uint32_t len = std::min(A.every, A.nsteps - s0);
-            load_ckpt(c, *A.ckpt[seg]);
-            for (uint32_t t = 0; t < len; ++t) pbf_step_taped(c, A.tape[t]->ptrs);
-            for (int t = (int)len - 1; t >= 0; --t) reverse_tick(c, A, *A.tape[t]);
+            auto enqueue = [&]() {
+                load_ckpt(c, *A.ckpt[seg]);
+                for (uint32_t t = 0; t < len; ++t) pbf_step_taped(c, A.tape[t]->ptrs);
+                for (int t = (int)len - 1; t >= 0; --t) reverse_tick(c, A, *A.tape[t]);
+            };
+            if (!graphs) {
+                enqueue();
+            } else if (!A.seg_exec[seg]) {
+                VTP_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+                hipGraph_t g = nullptr;
+                try {
+                    enqueue();   // the host-side ping-pong swaps happen here, once per reverse tick
+                } catch (...) {
+                    (void)hipStreamEndCapture(c.stream, &g);   // leave the stream usable
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                VTP_HIP(hipStreamEndCapture(c.stream, &g));
+                hipError_t e = hipGraphInstantiate(&A.seg_exec[seg], g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                VTP_HIP(e);
+                VTP_HIP(hipGraphLaunch(A.seg_exec[seg], c.stream));
+            } else {
+                VTP_HIP(hipGraphLaunch(A.seg_exec[seg], c.stream));
+                for (uint32_t t = 0; t < len; ++t) {   // the swaps the captured reverse ticks made
+                    std::swap(A.xb.p, A.xb2.p);
+                    std::swap(A.vb.p, A.vb2.p);
+                }
+            }
         }
         VTP_HIP(hipGetLastError());
         // adjoints are now in the slot order of the rollout's first state

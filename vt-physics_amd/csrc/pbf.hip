@@ -137,10 +137,14 @@ __global__ __launch_bounds__(256, VTP_PASS_MIN_WAVES) void k_pbf_dx(const float4
                                                 float4 *__restrict__ xo, float4 *__restrict__ v,
                                                 const int32_t *__restrict__ mat, const uint32_t *__restrict__ nbr,
                                                 const uint32_t *__restrict__ cnt, uint32_t n, uint32_t stride,
-                                                PbfK q) {
+                                                PbfK q, int copy_boundary) {
     const uint32_t g = slot_index(), i = g / L, sub = g % L;
     if (i >= n) return;
-    if (mat[i] != VTP_MAT_FLUID) return;
+    if (mat[i] != VTP_MAT_FLUID) {
+        // a taped tick chains distinct buffers, which need the static particles too
+        if (copy_boundary && sub == 0) xo[i] = x[i];
+        return;
+    }
     const float4 xi = x[i];
     const float li = xl[i].w;
     const uint32_t nn = cnt[i] & ~kDeadBit;
@@ -280,19 +284,29 @@ static void pbf_enqueue_step(Ctx &c, PbfTapeStep *tape = nullptr) {
     hipLaunchKernelGGL(k_pbf_predict_hash, dim3(nb), dim3(256), 0, c.stream, c.X[0].p, c.V[0].p, c.mat.p,
                        c.scene.p, n, q, c.grid, u.key[0].p, u.val[0].p, c.status.p);
     c.timer.end(c.stream);
+    // A taped tick writes its permutation, lists and per-iteration positions
+    // straight into the tape: the search's buffers are lent to the tape's for
+    // the tick, and the iterations chain the tape's position buffers.
+    struct Lend {
+        Ugns &u;
+        PbfTapeStep *t;
+        void swap() {
+            if (!t) return;
+            std::swap(u.perm.p, t->perm);
+            std::swap(u.nbr.p, t->nbr);
+            std::swap(u.cnt.p, t->cnt);
+        }
+        ~Lend() { swap(); }
+    } lend{u, tape};
+    lend.swap();
     u.sort_and_rank(c.orig.p, c.stream, c.timer);
     c.timer.begin(c.stream, "pbf_reorder");
-    common_reorder(c, c.X[0].p, c.X[1].p, c.X[2].p, c.V[0].p, c.V[1].p, nullptr);
+    float4 *const x0 = tape ? tape->x[0] : c.X[1].p;
+    common_reorder(c, c.X[0].p, x0, tape ? nullptr : c.X[2].p, c.V[0].p, c.V[1].p, nullptr);
     c.timer.end(c.stream);
-    u.build_neighbors(c.X[1].p, c.stream, c.timer);
-    if (tape) {
-        VTP_HIP(hipMemcpyAsync(tape->perm, u.perm.p, n * 4, hipMemcpyDeviceToDevice, c.stream));
-        VTP_HIP(hipMemcpyAsync(tape->orig, c.orig.p, n * 4, hipMemcpyDeviceToDevice, c.stream));
-        VTP_HIP(hipMemcpyAsync(tape->cnt, u.cnt.p, n * 4, hipMemcpyDeviceToDevice, c.stream));
-        VTP_HIP(hipMemcpyAsync(tape->nbr, u.nbr.p, (size_t)u.cap * u.stride * 4, hipMemcpyDeviceToDevice,
-                               c.stream));
-    }
-    float4 *cur = c.X[1].p, *nxt = c.X[2].p;
+    u.build_neighbors(x0, c.stream, c.timer);
+    if (tape) VTP_HIP(hipMemcpyAsync(tape->orig, c.orig.p, n * 4, hipMemcpyDeviceToDevice, c.stream));
+    float4 *cur = x0, *nxt = c.X[2].p;
     const uint32_t bl = c.block("pbf_lambda"), bd = c.block("pbf_dx"), bx = c.block("pbf_xsph");
     const uint32_t L = c.opt_lanes;
     for (uint32_t it = 0; it < c.p.iteration_num; ++it) {
@@ -300,14 +314,14 @@ static void pbf_enqueue_step(Ctx &c, PbfTapeStep *tape = nullptr) {
         float *lam = tape ? tape->lam[it] : c.lam.p;
         float *errc = tape ? tape->C[it] : c.err_c.p;
         float4 *sg = tape ? tape->sg[it] : nullptr;
-        if (tape) VTP_HIP(hipMemcpyAsync(tape->x[it], cur, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
+        if (tape) nxt = tape->x[it + 1];
         const dim3 gl(div_up(n * L, bl)), gd(div_up(n * L, bd));
 #define VTP_LAMBDA(MW, LL)                                                                                     \
     hipLaunchKernelGGL((k_pbf_lambda<MW, LL>), gl, dim3(bl), 0, c.stream, cur, c.mat.p, u.nbr.p, u.cnt.p, n,    \
                        u.stride, q, lam, c.dens.p, errc, sg, c.xl.p)
 #define VTP_DX(MW, LL)                                                                                         \
     hipLaunchKernelGGL((k_pbf_dx<MW, LL>), gd, dim3(bd), 0, c.stream, cur, c.xl.p, nxt, c.V[1].p, c.mat.p,    \
-                       u.nbr.p, u.cnt.p, n, u.stride, q)
+                       u.nbr.p, u.cnt.p, n, u.stride, q, tape ? 1 : 0)
         if (q.mass_weighted) { if (L == 4) VTP_LAMBDA(true, 4); else VTP_LAMBDA(true, 1); }
         else { if (L == 4) VTP_LAMBDA(false, 4); else VTP_LAMBDA(false, 1); }
         c.timer.end(c.stream);
@@ -319,11 +333,7 @@ static void pbf_enqueue_step(Ctx &c, PbfTapeStep *tape = nullptr) {
         c.timer.end(c.stream);
         std::swap(cur, nxt);
     }
-    if (tape) {
-        const uint32_t K = c.p.iteration_num;
-        VTP_HIP(hipMemcpyAsync(tape->x[K], cur, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
-        VTP_HIP(hipMemcpyAsync(tape->v_last, c.V[1].p, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
-    }
+    if (tape) VTP_HIP(hipMemcpyAsync(tape->v_last, c.V[1].p, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
     c.timer.begin(c.stream, "pbf_xsph");
     if (L == 4)
         hipLaunchKernelGGL(k_pbf_xsph<4>, dim3(div_up(n * 4, bx)), dim3(bx), 0, c.stream, cur, c.X[0].p, c.V[1].p,
@@ -345,6 +355,7 @@ void pbf_capture_reset(Ctx &c) {
     if (c.graph) (void)hipGraphDestroy(c.graph);
     c.graph_exec = nullptr;
     c.graph = nullptr;
+    ++c.capture_epoch;   // graphs captured elsewhere (the adjoint's) are stale as well
 }
 
 void pbf_step(Ctx &c) {

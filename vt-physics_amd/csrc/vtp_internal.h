@@ -177,6 +177,11 @@ struct PbfTapeStep {
     float4 *v_last;      // velocities entering XSPH
 };
 
+// Particle count below which neighbour passes take 4 lanes per particle:
+// with one lane, such a scene leaves most CUs idle behind each particle's
+// chain of dependent gathers.
+constexpr uint32_t kSmallScene = 1u << 18;
+
 struct Ctx;
 PbfK pbf_constants(const Ctx &c);
 void pbf_step_taped(Ctx &c, PbfTapeStep &tape);
@@ -242,6 +247,7 @@ struct Ctx {
     DevBuf<float4> xl;          // (x, lambda) of the current constraint iteration
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    uint64_t capture_epoch = 0;   // bumped whenever captured kernel arguments go stale
     // PBF adjoint state (checkpoints and tapes), owned here
     std::shared_ptr<void> adj_state;
     bool lists_stale = false;   // neighbour lists do not match X[0] (after a rollout backward)
