@@ -280,6 +280,9 @@ def main():
     step_bytes = pbf_step_bytes(n, nf, cells * wl.scenes(), params.iteration_num, passes)
     st = ctx.stats()
     import numpy as np
+    # ticks build fluid lists only; one more tick with every list for the statistics
+    ctx.set_option("static_lists", 1)
+    ctx.step(1)
     num = ctx.debug_neighbors()["neighbor_num"].astype(np.int64)   # self included, reference layout
     nbr_counts = {"mean": round(float(num[num > 0].mean()), 2), "max": int(num.max()),
                   "at_cap_fraction": round(float((num >= params.max_neighbor_num).mean()), 6)}

@@ -309,7 +309,11 @@ vtp_status vtp_load_state(vtp_ctx *ctx, const void *host_src, size_t bytes);
  * switches to the reference's cell order once many lists reach the cap;
  * 1 = always rows; 2 = always the reference's order.  The neighbour sets are
  * the reference's either way; the order inside a list, and so the FP32
- * rounding of neighbour sums, depends on the walk). */
+ * rounding of neighbour sums, depends on the walk),
+ * "static_lists" (PBF: 0 = default, a tick's search builds the lists of fluid
+ * particles only, since no PBF kernel reads a static particle's list after
+ * initialize; 1 = every list, as the reference, which vtp_debug_neighbors
+ * needs after a tick). */
 vtp_status vtp_set_option(vtp_ctx *ctx, const char *key, double value);
 
 /* Per-stage device time (ms, summed over launches) and launch counts,

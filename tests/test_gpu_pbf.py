@@ -122,3 +122,23 @@ def test_pbf_autotune_keeps_results_bit_identical():
     np.testing.assert_array_equal(a.read("pos"), b.read("pos"))
     np.testing.assert_array_equal(a.read("vel"), b.read("vel"))
     assert b.stats()["steps"] == a.stats()["steps"] == 5
+
+
+def test_pbf_static_lists_only_change_what_is_exported():
+    """PBF ticks skip the lists of wall particles (no tick kernel reads them):
+    the fluid results equal those of a context that builds every list, and
+    only the complete search can be exported in the reference's layout."""
+    wl = scenes.cfg0_pbf(n_side=(16, 16, 16))
+    a, _ = _pair(wl)
+    b, _ = _pair(wl)
+    b.set_option("static_lists", 1)
+    a.step(6)
+    b.step(6)
+    np.testing.assert_array_equal(a.read("pos"), b.read("pos"))
+    np.testing.assert_array_equal(a.read("vel"), b.read("vel"))
+    with pytest.raises(vtp.VtpError, match="static_lists"):
+        a.debug_neighbors()
+    g = b.debug_neighbors()
+    mat = b.read("mat")
+    walls = np.flatnonzero(mat != vtp.MAT_FLUID)
+    assert (g["neighbor_num"][walls] > 1).all()   # wall lists present again

@@ -564,6 +564,10 @@ extern "C" vtp_status vtp_debug_neighbors(vtp_ctx *ctx, uint32_t *neighbors, uin
     if (!c.initialized) return ctx_fail(c, VTP_ESTATE, "vtp_debug_neighbors: not initialized");
     if (c.lists_stale)
         return ctx_fail(c, VTP_ESTATE, "vtp_debug_neighbors: lists belong to a replaced state; tick first");
+    if (!c.ugns.static_rows)
+        return ctx_fail(c, VTP_ESTATE,
+                        "vtp_debug_neighbors: the last search built fluid lists only (PBF ticks); set option "
+                        "static_lists=1 and tick for the reference's complete lists");
     return guarded(c, [&]() -> vtp_status {
         const uint32_t n = c.n, M = c.p.max_neighbor_num;
         if (neighbors || neighbor_num) {
@@ -625,6 +629,9 @@ extern "C" vtp_status vtp_set_option(vtp_ctx *ctx, const char *key, double value
         } else if (k == "lanes") {
             if (value != 1.0 && value != 4.0) throw std::runtime_error("lanes must be 1 or 4");
             c.opt_lanes = (uint32_t)value;
+            pbf_capture_reset(c);
+        } else if (k == "static_lists") {
+            c.opt_static_lists = value != 0.0;
             pbf_capture_reset(c);
         } else if (k == "neighbor_walk") {
             if (value != 0.0 && value != 1.0 && value != 2.0)

@@ -304,7 +304,8 @@ static void pbf_enqueue_step(Ctx &c, PbfTapeStep *tape = nullptr) {
     float4 *const x0 = tape ? tape->x[0] : c.X[1].p;
     common_reorder(c, c.X[0].p, x0, tape ? nullptr : c.X[2].p, c.V[0].p, c.V[1].p, nullptr);
     c.timer.end(c.stream);
-    u.build_neighbors(x0, c.stream, c.timer);
+    // no kernel of a PBF tick (nor its adjoint) walks a static particle's list
+    u.build_neighbors(x0, c.stream, c.timer, c.opt_static_lists ? nullptr : c.mat.p);
     if (tape) VTP_HIP(hipMemcpyAsync(tape->orig, c.orig.p, n * 4, hipMemcpyDeviceToDevice, c.stream));
     float4 *cur = x0, *nxt = c.X[2].p;
     const uint32_t bl = c.block("pbf_lambda"), bd = c.block("pbf_dx"), bx = c.block("pbf_xsph");

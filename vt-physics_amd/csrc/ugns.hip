@@ -304,7 +304,8 @@ __global__ __launch_bounds__(kNbrBuildThreads) void k_build_neighbors(const floa
                                                                       uint32_t *__restrict__ nbr,
                                                                       uint32_t *__restrict__ cnt,
                                                                       uint32_t *__restrict__ status,
-                                                                      uint32_t walk) {
+                                                                      uint32_t walk,
+                                                                      const int32_t *__restrict__ fluid_only) {
     extern __shared__ uint32_t s_list[];
     const uint32_t i = slot_index();
     // uniform over the grid; walk = 1 / 2 forces the row / reference walk
@@ -320,6 +321,8 @@ __global__ __launch_bounds__(kNbrBuildThreads) void k_build_neighbors(const floa
         const uint32_t k = key[i];
         if (k >= g.dead_key) {
             cnt[i] = kDeadBit;
+        } else if (fluid_only && fluid_only[i] != VTP_MAT_FLUID) {
+            cnt[i] = 0;   // a static particle whose list no kernel of this tick reads
         } else {
             const uint32_t gxy = (uint32_t)(g.gx * g.gy);
             const uint32_t scene = k / g.cells;
@@ -495,12 +498,13 @@ void Ugns::sort_and_rank(const uint32_t *orig_old, hipStream_t s, StageTimer &tm
     tm.end(s);
 }
 
-void Ugns::build_neighbors(const float4 *x, hipStream_t s, StageTimer &tm) {
+void Ugns::build_neighbors(const float4 *x, hipStream_t s, StageTimer &tm, const int32_t *fluid_only) {
     if (!status) throw std::runtime_error("neighbour build without status words");
+    static_rows = fluid_only == nullptr;
     tm.begin(s, "ugns_neighbors");
     const size_t lds = (size_t)std::min(cap, kNbrStage) * kNbrBuildThreads * sizeof(uint32_t);
     hipLaunchKernelGGL(k_build_neighbors<true>, dim3(blocks_for(n, kNbrBuildThreads)), dim3(kNbrBuildThreads), lds,
-                       s, x, key[sorted].p, start.p, end.p, g, n, stride, cap, nbr.p, cnt.p, status, walk);
+                       s, x, key[sorted].p, start.p, end.p, g, n, stride, cap, nbr.p, cnt.p, status, walk, fluid_only);
     tm.end(s);
 }
 

@@ -117,10 +117,13 @@ struct Ugns {
     DevBuf<uint32_t> nbr, cnt;      // ELL lists nbr[t*stride + slot], count word per slot
     uint32_t *status = nullptr;     // the context's status words (cap counts)
     uint32_t walk = 0;              // list walk: 0 adaptive, 1 row walk, 2 reference walk
+    bool static_rows = true;        // the last build holds the lists of static particles too
 
     void allocate(uint32_t n, const Grid &g, uint32_t max_neighbors);
     void sort_and_rank(const uint32_t *orig_old, hipStream_t s, StageTimer &tm);
-    void build_neighbors(const float4 *x, hipStream_t s, StageTimer &tm);
+    // fluid_only = per-slot materials: the lists of non-fluid particles are
+    // left empty (count 0); nullptr builds every list, as the reference does
+    void build_neighbors(const float4 *x, hipStream_t s, StageTimer &tm, const int32_t *fluid_only = nullptr);
     void export_reference(const uint32_t *orig, uint32_t *d_lists, uint32_t *d_num, hipStream_t s);
     const uint32_t *sorted_key() const { return key[sorted].p; }
 };
@@ -225,6 +228,7 @@ struct Ctx {
     uint32_t opt_block = 256;
     uint32_t opt_chunk = 0;   // 0 = adaptive
     uint32_t opt_lanes = 1;   // lanes per particle in the PBF neighbour passes (1 or 4)
+    bool opt_static_lists = false;   // PBF ticks also build the (unread) lists of static particles
     StageTimer timer;
 
     // attach-order tables (static)
