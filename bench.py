@@ -205,6 +205,43 @@ def cpu_baseline(wl, params, budget_s=20.0):
             "sample": f"FP64 oracle, {wl.name}: {steps} ticks after 1 untimed tick ({el:.1f} s)"}
 
 
+STAGE_KERNEL = {"pbf_lambda": "k_pbf_lambda", "pbf_dx": "k_pbf_dx", "pbf_xsph": "k_pbf_xsph",
+                "ugns_neighbors": "k_build_neighbors"}
+TRAFFIC_JSON = ROOT / "profiles" / "pbf_dam_break_1m_traffic.json"
+
+
+def traffic_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, out=TRAFFIC_JSON):
+    """HBM bytes per launch of the neighbour-pass kernels from two rocprofv3
+    --pmc runs (FETCH_SIZE, WRITE_SIZE, KiB per dispatch), each scaled by what
+    the same counter reads for a plain device copy of known size."""
+    import csv
+    from collections import defaultdict
+
+    def per_kernel(path, counter):
+        acc = defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] == counter:
+                acc[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+        return {k: sum(v) / len(v) for k, v in acc.items()}
+
+    cf = max(per_kernel(copy_fetch_csv, "FETCH_SIZE").values())
+    cw = max(per_kernel(copy_write_csv, "WRITE_SIZE").values())
+    copy_kib = (1 << 30) / 1024   # calibrate_copy: 1 GiB read and 1 GiB written per dispatch
+    fetch_scale, write_scale = copy_kib / cf, copy_kib / cw
+    fetch, write = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
+    kernels = {}
+    for stage, kname in STAGE_KERNEL.items():
+        fk = [v for k, v in fetch.items() if kname in k]
+        wk = [v for k, v in write.items() if kname in k]
+        if fk and wk:
+            rb, wb = max(fk) * 1024 * fetch_scale, max(wk) * 1024 * write_scale
+            kernels[stage] = {"read_bytes": round(rb), "write_bytes": round(wb), "bytes": round(rb + wb)}
+    doc = {"workload": "pbf_dam_break_1m", "fetch_scale": round(fetch_scale, 4), "write_scale": round(write_scale, 4),
+           "kernels": kernels}
+    Path(out).write_text(json.dumps(doc, indent=1) + "\n")
+    print(json.dumps(doc))
+
+
 def calibrate_copy(torch, nbytes=1 << 30, reps=5):
     """Plain device-to-device copies of a known size: the reference reading for
     scaling the FETCH_SIZE / WRITE_SIZE counters of the kernels above."""
@@ -230,7 +267,12 @@ def main():
     ap.add_argument("--only-headline", action="store_true", help="skip the DFSPH / batched / adjoint / IMM-CT legs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 1 GiB device copies (counter calibration for the traffic figure)")
+    ap.add_argument("--traffic-summary", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
+                    help="turn rocprofv3 --pmc counter files into " + str(TRAFFIC_JSON.relative_to(ROOT)))
     args = ap.parse_args()
+    if args.traffic_summary:
+        traffic_summary(*args.traffic_summary)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1 "
               "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
@@ -275,7 +317,13 @@ def main():
         ach = kb / (dom_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
-                "kernel_ms": round(dom_ms, 5)}
+                "kernel_ms": round(dom_ms, 5), "algorithmic_bytes": kb}
+        if TRAFFIC_JSON.exists():   # counter-measured HBM bytes per launch (rocprofv3 --pmc, calibrated)
+            t = json.loads(TRAFFIC_JSON.read_text())["kernels"].get(dom)
+            if t:
+                roof["traffic"] = t["bytes"]
+                roof["traffic_GBps"] = round(t["bytes"] / (dom_ms * 1e-3) / 1e9, 1)
+                roof["traffic_source"] = str(TRAFFIC_JSON.relative_to(ROOT))
     passes = 3 if cells * wl.scenes() >= (1 << 16) else 2
     step_bytes = pbf_step_bytes(n, nf, cells * wl.scenes(), params.iteration_num, passes)
     st = ctx.stats()

@@ -136,16 +136,16 @@ __global__ __launch_bounds__(256) void k_df_precompute(const float4 *__restrict_
                                                        const uint32_t *__restrict__ nbr,
                                                        const uint32_t *__restrict__ cnt, uint32_t n,
                                                        uint32_t stride, DfK q, float *__restrict__ cr,
-                                                       float *__restrict__ alpha, float4 *__restrict__ xv) {
+                                                       float *__restrict__ alpha, float4 *__restrict__ g1) {
     uint32_t i = slot_index();
     if (i >= n) return;
     const float4 vi = v[i];
     const float4 xi = x[i];
     vadv[i] = vi;
-    xv[i] = make_float4(xi.x, xi.y, xi.z, vi.w);   // (x, V) for the v* passes of this tick
     if (mat[i] != VTP_MAT_FLUID) {
         cr[i] = 0.0f;
         alpha[i] = 0.0f;
+        g1[i] = make_float4(0, 0, 0, 0);
         return;
     }
     const uint32_t w = cnt[i];
@@ -169,6 +169,7 @@ __global__ __launch_bounds__(256) void k_df_precompute(const float4 *__restrict_
     float a = (a1x * a1x + a1y * a1y + a1z * a1z) / xi.w + a2;
     cr[i] = c;
     alpha[i] = fmaxf(a, 1e-6f);
+    g1[i] = make_float4(a1x, a1y, a1z, 0.0f);   // sum_j V_j gradW_ij, reused by every v* update of the tick
 }
 
 // ------------------------------------------------------------------ pressure solve
@@ -195,7 +196,7 @@ __global__ __launch_bounds__(256) void k_df_delta_kappa(const float4 *__restrict
                                                         const uint32_t *__restrict__ cnt, uint32_t n,
                                                         uint32_t stride, Cubic k, const float *__restrict__ cr,
                                                         const float *__restrict__ alpha, float *__restrict__ delta,
-                                                        float *__restrict__ kappa, Loop lp) {
+                                                        float *__restrict__ kappa, float4 *__restrict__ xk, Loop lp) {
     if (lp.status[kStDone]) return;
     const uint32_t i = slot_index();
     const float dt = lp.sc->dt;
@@ -218,11 +219,16 @@ __global__ __launch_bounds__(256) void k_df_delta_kappa(const float4 *__restrict
                 });
             float dl = fmaxf(cr[i] - 1.0f + d * dt, 0.0f);
             float inv_dt = 1.0f / dt;
+            const float Vi = v[i].w;
+            const float ki = dl / alpha[i] * (inv_dt * inv_dt) / Vi;
             delta[i] = dl;
-            kappa[i] = dl / alpha[i] * (inv_dt * inv_dt) / v[i].w;
+            kappa[i] = ki;
+            xk[i] = make_float4(xi.x, xi.y, xi.z, Vi * ki);
             contrib = (double)dl * lp.obj_w[obj_of[i]];
         } else {
             kappa[i] = 0.0f;
+            const float4 xi = x[i];
+            xk[i] = make_float4(xi.x, xi.y, xi.z, 0.0f);
         }
     }
     // residual = sum_i delta_i w_obj(i) (the mean over fluid objects of the
@@ -242,9 +248,13 @@ __global__ __launch_bounds__(256) void k_df_delta_kappa(const float4 *__restrict
 }
 
 // vf_update_vel_adv_from_kappa_* (DFSPHCudaImpl.cu:198-220, 369-391):
-//   v*_i -= dt V_i V_j / m_i (kappa_i + kappa_j) gradW_ij over all neighbours.
+//   v*_i -= dt V_i V_j / m_i (kappa_i + kappa_j) gradW_ij over all neighbours,
+// evaluated as dt V_i / m_i (kappa_i g1_i + sum_j (V_j kappa_j) gradW_ij) with
+// g1_i = sum_j V_j gradW_ij from the precompute pass (positions are fixed
+// during the solve): a neighbour is one 16-byte (x_j, V_j kappa_j) gather.
 __global__ __launch_bounds__(256) void k_df_update_vadv(const float4 *__restrict__ x, const float4 *__restrict__ v,
-                                                        const float4 *__restrict__ xv,
+                                                        const float4 *__restrict__ xk,
+                                                        const float4 *__restrict__ g1,
                                                         float4 *__restrict__ vadv, const int32_t *__restrict__ mat,
                                                         const uint32_t *__restrict__ nbr,
                                                         const uint32_t *__restrict__ cnt, uint32_t n,
@@ -259,16 +269,19 @@ __global__ __launch_bounds__(256) void k_df_update_vadv(const float4 *__restrict
     const float ki = kappa[i];
     const uint32_t nn = cnt[i] & ~kDeadBit;
     float ax = 0, ay = 0, az = 0;
-    struct D { float4 x; float k; };
     for_each_neighbor(
-        nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(xv, j), ldg(kappa, j)}; },   // (x_j, V_j), kappa_j
-        [&](const D &e) {
-            float dx = xi.x - e.x.x, dy = xi.y - e.x.y, dz = xi.z - e.x.z;
-            float s = cubic_grad_factor(dx * dx + dy * dy + dz * dz, k) * e.x.w * (ki + e.k);
+        nbr, stride, i, nn, [&](uint32_t j) { return ldg(xk, j); },   // (x_j, V_j kappa_j)
+        [&](const float4 &e) {
+            float dx = xi.x - e.x, dy = xi.y - e.y, dz = xi.z - e.z;
+            float s = cubic_grad_factor(dx * dx + dy * dy + dz * dz, k) * e.w;
             ax = fmaf(dx, s, ax);
             ay = fmaf(dy, s, ay);
             az = fmaf(dz, s, az);
         });
+    const float4 g = g1[i];
+    ax = fmaf(ki, g.x, ax);
+    ay = fmaf(ki, g.y, ay);
+    az = fmaf(ki, g.z, az);
     float c = dt * Vi / xi.w;
     float4 a = vadv[i];
     a.x -= c * ax;
@@ -477,12 +490,11 @@ void df_pressure_solve(Ctx &c, const float4 *x, const float4 *v, int which) {
             c.timer.begin(c.stream, name_dk);
             hipLaunchKernelGGL(k_df_delta_kappa, dim3(div_up(n, bd)), dim3(bd), 0, c.stream, x, v, c.vadv.p, c.mat.p,
                                c.obj_of.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, c.cr.p, c.alpha.p, c.delta.p,
-                               c.kappa.p, lp);
+                               c.kappa.p, c.xk.p, lp);
             c.timer.end(c.stream);
             c.timer.begin(c.stream, name_up);
-            hipLaunchKernelGGL(k_df_update_vadv, dim3(div_up(n, bu)), dim3(bu), 0, c.stream, x, v, c.xv.p, c.vadv.p,
-                               c.mat.p,
-                               u.nbr.p, u.cnt.p, n, u.stride, c.kern, c.kappa.p, lp);
+            hipLaunchKernelGGL(k_df_update_vadv, dim3(div_up(n, bu)), dim3(bu), 0, c.stream, x, v, c.xk.p, c.g1.p,
+                               c.vadv.p, c.mat.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, c.kappa.p, lp);
             c.timer.end(c.stream);
             hipLaunchKernelGGL(k_df_decide, dim3(1), dim3(256), 0, c.stream, lp, div_up(n, bd), thr, cap, word,
                                bit, which);
@@ -509,7 +521,8 @@ void dfsph_allocate_common(Ctx &c) {
     c.vadv.alloc(n);
     c.acc.alloc(n);
     c.normal.alloc(n);
-    c.xv.alloc(n);
+    c.xk.alloc(n);
+    c.g1.alloc(n);
     VTP_HIP(hipMemsetAsync(c.cr.p, 0, c.cr.bytes(), c.stream));
     VTP_HIP(hipMemsetAsync(c.alpha.p, 0, c.alpha.bytes(), c.stream));
     VTP_HIP(hipMemsetAsync(c.delta.p, 0, c.delta.bytes(), c.stream));
@@ -587,7 +600,7 @@ void dfsph_step(Ctx &c) {
     const uint32_t bp = c.block("df_precompute");
     c.timer.begin(c.stream, "df_precompute");
     hipLaunchKernelGGL(k_df_precompute, dim3(div_up(n, bp)), dim3(bp), 0, c.stream, x, v, c.vadv.p, c.mat.p, u.nbr.p,
-                       u.cnt.p, n, u.stride, q, c.cr.p, c.alpha.p, c.xv.p);
+                       u.cnt.p, n, u.stride, q, c.cr.p, c.alpha.p, c.g1.p);
     c.timer.end(c.stream);
     df_pressure_solve(c, x, v, 0);
     c.timer.begin(c.stream, "df_apply_pressure");

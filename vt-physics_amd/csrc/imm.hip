@@ -189,19 +189,16 @@ __global__ __launch_bounds__(256) void k_imm_precompute(const float4 *__restrict
                                                         const uint32_t *__restrict__ cnt, uint32_t n,
                                                         uint32_t stride, ImmK q, const float *__restrict__ fpoly,
                                                         float *__restrict__ cr, float *__restrict__ alpha,
-                                                        float *__restrict__ cd, float4 *__restrict__ xv) {
+                                                        float *__restrict__ cd, float4 *__restrict__ g1) {
     uint32_t i = slot_index();
     if (i >= n) return;
     const float4 vi = v[i];
     vadv[i] = vi;
-    {
-        const float4 xs = x[i];
-        xv[i] = make_float4(xs.x, xs.y, xs.z, vi.w);   // (x, V) for the v* passes of this tick
-    }
     const int32_t mi = mat[i];
     if (mi != VTP_MAT_FLUID) {
         cr[i] = 0.0f;
         alpha[i] = 0.0f;
+        g1[i] = make_float4(0, 0, 0, 0);
         if (q.ct) cd[i] = q.cd0;
         return;
     }
@@ -227,6 +224,7 @@ __global__ __launch_bounds__(256) void k_imm_precompute(const float4 *__restrict
         });
     cr[i] = c;
     alpha[i] = fmaxf((a1x * a1x + a1y * a1y + a1z * a1z) / xi.w + a2, 1e-6f);
+    g1[i] = make_float4(a1x, a1y, a1z, 0.0f);   // sum_j V_j gradW_ij for the v* updates
     if (q.ct) cd[i] = q.cd0 + (1.0f - q.cd0) * b;
 }
 
@@ -824,7 +822,7 @@ static void imm_step_impl(Ctx &c) {
     const uint32_t nbB = div_up(n, B);
     c.timer.begin(c.stream, "imm_precompute");
     hipLaunchKernelGGL(k_imm_precompute, dim3(nbB), dim3(B), 0, c.stream, x, v, c.vadv.p, c.mat.p, u.nbr.p, u.cnt.p, n,
-                       st, q, c.fpoly.p, c.cr.p, c.alpha.p, c.cd.p, c.xv.p);
+                       st, q, c.fpoly.p, c.cr.p, c.alpha.p, c.cd.p, c.g1.p);
     c.timer.end(c.stream);
     df_pressure_solve(c, x, v, 0);
     c.timer.begin(c.stream, "imm_pressure_to_phases");

@@ -265,7 +265,8 @@ struct Ctx {
     DevBuf<double> partial;                  // residual partial per workgroup
     uint32_t iter_hint_div = 1, iter_hint_inc = 1;
     DevBuf<float4> vadv, acc, normal;
-    DevBuf<float4> xv;   // (x, V) of the current tick, gathered by the v* passes
+    DevBuf<float4> xk;   // (x, V kappa) of the current solver iteration, gathered by the v* pass
+    DevBuf<float4> g1;   // sum_j V_j gradW_ij of the tick (positions are fixed during the solves)
     // IMM (phase-major: [k*stride + slot])
     uint32_t P = 0;
     DevBuf<float> frac[2], frac_in, frac_out;
