@@ -25,18 +25,20 @@ namespace vtp {
 
 void pbf_step(Ctx &c);
 
-// Hessian-vector product H(r) a of the cubic spline, with gradW = f r.
+// Hessian-vector product H(r) a of the cubic spline, with gradW = f r:
+// H = f I + d r r^T, d = (1/r) df/dr.  In the truncated-power form used by the
+// forward passes, |gradW| = G = gt t^2 + gu u^2 with dt/dr = du/dr = -1/h
+// where t, u > 0, so dG/dr = -(2/h)(gt t + gu u) and
+// d = (dG/dr - G/r) / r^2.
 __device__ __forceinline__ void kernel_f_d(float r2, const Cubic &k, float &f, float &d) {
-    float rinv = __frsqrt_rn(r2);
-    float r = r2 * rinv;
-    float q = r * k.inv_h;
-    bool inner = q <= 0.5f;
-    float t = 1.0f - q;
-    float g = inner ? 6.0f * q * fmaf(3.0f, q, -2.0f) : -6.0f * t * t;
-    float gp = inner ? 6.0f * fmaf(6.0f, q, -2.0f) : 12.0f * t;
-    bool ok = q >= 1e-6f && q <= 1.0f;
-    f = ok ? k.grad_scale * g * rinv : 0.0f;
-    d = ok ? k.grad_scale * rinv * rinv * (gp * k.inv_h - g * rinv) : 0.0f;
+    const float rinv = cubic_rinv(r2);
+    const float r = r2 * rinv;
+    const float t = unit_clamp(fmaf(-r, k.inv_h, 1.0f)), u = unit_clamp(fmaf(-r, k.inv_h, 0.5f));
+    const float G = fmaf(t * t, k.gt, (u * u) * k.gu);
+    const float dG = -2.0f * k.inv_h * fmaf(t, k.gt, u * k.gu);
+    const bool ok = r2 >= cubic_r2_lo(k);
+    f = ok ? G * rinv : 0.0f;
+    d = ok ? rinv * rinv * (dG - G * rinv) : 0.0f;
 }
 
 __global__ void k_adj_mat_from_orig(const uint32_t *__restrict__ orig, const int32_t *__restrict__ mat_attach,

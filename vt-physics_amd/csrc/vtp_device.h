@@ -87,6 +87,8 @@ struct Cubic {
     float h, inv_h, sigma;      // sigma = 8/(pi h^3)
     float grad_scale;           // sigma / h
     float w0;                   // W(0) = sigma
+    float wt, wu;               // 2 sigma, -8 sigma: W = wt t^3 + wu u^3
+    float gt, gu;               // -6 sigma/h, 24 sigma/h: |gradW| = gt t^2 + gu u^2
 };
 
 __host__ inline Cubic make_cubic(float h) {
@@ -97,56 +99,58 @@ __host__ inline Cubic make_cubic(float h) {
     c.sigma = 8.0f / pi / (h * h * h);
     c.grad_scale = c.sigma / h;
     c.w0 = c.sigma;
+    c.wt = 2.0f * c.sigma;
+    c.wu = -8.0f * c.sigma;
+    c.gt = -6.0f * c.grad_scale;
+    c.gu = 24.0f * c.grad_scale;
     return c;
 }
 
-// The branches are evaluated and selected (no divergent branch in the
-// neighbour loops), sharing t = max(1 - q, 0): the outer branch is then zero
-// past q = 1 by itself (W and its derivative vanish at q = 1), so the q > 1
-// test costs nothing.  r = r2 * rsqrt(r2) (one transcendental; a correctly
-// rounded sqrt costs a dozen instructions and changes W by < 1e-7 relative).
-//   W:    q <= 0.5: sigma(1 + q^2 (6q - 6));   q <= 1: 2 sigma (1-q)^3
-//   grad: q <= 0.5: 6q(3q - 2) sigma/h;         q <= 1: -6 (1-q)^2 sigma/h, times r_vec/r
+// Truncated-power form of the cubic spline.  With q = r/h, t = (1 - q)_+ and
+// u = (1/2 - q)_+,
+//   W(q)  = sigma (6q^3 - 6q^2 + 1) = 2 sigma (t^3 - 4 u^3)        (q <= 1/2)
+//         = 2 sigma (1 - q)^3       = 2 sigma t^3                  (1/2 < q <= 1)
+//   dW/dq = 6 sigma q (3q - 2)      = 2 sigma (-3 t^2 + 12 u^2)    (q <= 1/2)
+//         = -6 sigma (1 - q)^2      = 2 sigma (-3 t^2)             (1/2 < q <= 1)
+// and both vanish for q > 1, so one straight-line expression covers the
+// reference's branches (KernelFunctions.cuh:13-56) with no compare or select.
+// t and u are fma(-r, 1/h, c) clamped to [0, 1] (1 - q and 1/2 - q are at
+// most 1 for q >= 0), which the compiler folds into the fma's clamp bit.
+// r = r2 * rsqrt(r2): one transcendental (a correctly rounded sqrt costs a
+// dozen instructions and changes W by < 1e-7 relative).
 // Squared-distance bound of the reference's q < 1e-6 gradient cut-off.
 __device__ __forceinline__ float cubic_r2_lo(const Cubic &k) { return (1e-6f * k.h) * (1e-6f * k.h); }
 
-// rsqrt of max(r2, FLT_MIN): finite for a coincident pair, whose q is then 0
+// rsqrt of max(r2, FLT_MIN): finite for a coincident pair, whose r is then 0
 // (W = sigma), so no neighbour loop branches on r2 == 0.  Identical to
 // rsqrt(r2) for every normal r2.
 __device__ __forceinline__ float cubic_rinv(float r2) { return __frsqrt_rn(fmaxf(r2, 1.17549435e-38f)); }
 
+__device__ __forceinline__ float unit_clamp(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+
 __device__ __forceinline__ float cubic_w(float r2, const Cubic &k) {
-    const float q = (r2 * cubic_rinv(r2)) * k.inv_h;
-    const float t = fmaxf(1.0f - q, 0.0f);
-    const float wi = fmaf(q * q, fmaf(q, 6.0f, -6.0f), 1.0f);
-    const float wo = 2.0f * (t * t) * t;
-    return k.sigma * (q <= 0.5f ? wi : wo);
+    const float r = r2 * cubic_rinv(r2);
+    const float t = unit_clamp(fmaf(-r, k.inv_h, 1.0f)), u = unit_clamp(fmaf(-r, k.inv_h, 0.5f));
+    return fmaf((t * t) * t, k.wt, ((u * u) * u) * k.wu);
 }
 
 // Scalar factor f such that gradW(r_vec) = f * r_vec (zero for q < 1e-6).
 __device__ __forceinline__ float cubic_grad_factor(float r2, const Cubic &k) {
     const float rinv = cubic_rinv(r2);
-    const float q = (r2 * rinv) * k.inv_h;
-    const float t = fmaxf(1.0f - q, 0.0f);
-    const float gi = q * fmaf(q, 18.0f, -12.0f);
-    const float go = -6.0f * (t * t);
-    const float f = k.grad_scale * (q <= 0.5f ? gi : go) * rinv;
+    const float r = r2 * rinv;
+    const float t = unit_clamp(fmaf(-r, k.inv_h, 1.0f)), u = unit_clamp(fmaf(-r, k.inv_h, 0.5f));
+    const float f = fmaf(t * t, k.gt, (u * u) * k.gu) * rinv;
     return r2 >= cubic_r2_lo(k) ? f : 0.0f;
 }
 
 // W and the gradient factor together (one rsqrt), straight-line code.
 __device__ __forceinline__ void cubic_w_grad(float r2, const Cubic &k, float &w, float &f) {
     const float rinv = cubic_rinv(r2);
-    const float q = (r2 * rinv) * k.inv_h;
-    const float t = fmaxf(1.0f - q, 0.0f);
-    const float t2 = t * t;
-    const bool inner = q <= 0.5f;
-    const float wi = fmaf(q * q, fmaf(q, 6.0f, -6.0f), 1.0f);
-    const float wo = 2.0f * t2 * t;
-    const float gi = q * fmaf(q, 18.0f, -12.0f);
-    const float go = -6.0f * t2;
-    w = k.sigma * (inner ? wi : wo);   // r2 == 0: q = 0, W = sigma
-    const float g = k.grad_scale * (inner ? gi : go) * rinv;
+    const float r = r2 * rinv;
+    const float t = unit_clamp(fmaf(-r, k.inv_h, 1.0f)), u = unit_clamp(fmaf(-r, k.inv_h, 0.5f));
+    const float t2 = t * t, u2 = u * u;
+    w = fmaf(t2 * t, k.wt, (u2 * u) * k.wu);   // r2 == 0: r = 0, W = sigma
+    const float g = fmaf(t2, k.gt, u2 * k.gu) * rinv;
     f = r2 >= cubic_r2_lo(k) ? g : 0.0f;
 }
 
