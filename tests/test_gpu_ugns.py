@@ -135,3 +135,51 @@ def test_ugns_cap_reached_in_the_x_minus_one_and_x_cells(walk):
     assert (g["neighbor_num"] == 20).all()
     assert (g["neighbors"][0, 1:20] < 20).all()
     assert (g["neighbors"][25, 1:20] < 20).all()
+
+
+def _shell_points(h, lb, seed):
+    """Anchors on and next to cell faces, edges and corners, each with partners
+    at distances just below, at and just above h in axis, face-diagonal,
+    body-diagonal and random directions."""
+    rng = np.random.default_rng(seed)
+    hf = float(np.float32(h))
+    eps = float(np.spacing(np.float32(hf)))
+    offs = [0.0, eps, 0.5 * hf, hf - eps]
+    dirs = []
+    for v in [(1, 0, 0), (0, 1, 0), (0, 0, 1)]:
+        dirs += [v, tuple(-c for c in v)]
+    for sx in (1, -1):
+        for sy in (1, -1):
+            dirs += [(sx, sy, 0), (sx, 0, sy), (0, sx, sy)]
+            for sz in (1, -1):
+                dirs.append((sx, sy, sz))
+    dirs = [np.array(d, float) / np.linalg.norm(d) for d in dirs]
+    dirs += list(rng.normal(size=(10, 3)))
+    dirs = [d / np.linalg.norm(d) for d in dirs]
+    pts = []
+    for i in (2, 5, 8):
+        for j in (2, 5, 8):
+            for k in (2, 5, 8):
+                o = rng.integers(0, len(offs), 3)
+                a = np.array([lb + (i + 0.0) * hf + offs[o[0]], lb + j * hf + offs[o[1]], lb + k * hf + offs[o[2]]])
+                a = a.astype(np.float32).astype(float)
+                pts.append(a)
+                for d in (hf * (1 - 2.0 ** -20), hf, hf * (1 + 2.0 ** -20)):
+                    for u in dirs:
+                        pts.append(a + d * u)
+    return np.array(pts, dtype=np.float32)
+
+
+@pytest.mark.parametrize("walk", [1, 2])
+def test_ugns_pairs_at_the_cutoff_across_cell_faces_and_corners(walk):
+    # Pairs at distance h (and one float either side) whose partner sits in a
+    # face, edge or corner cell.  Each is decided by the FP32 test with every
+    # product rounded (a fused dx*dx + dy*dy moves pairs across the cut-off),
+    # and no pair in an edge or corner cell may be lost.
+    pos = _shell_points(0.04, 0.0, seed=19)
+    wl = scenes.Workload("shell", "PBF", scenes.pbf_config(radius=0.01, lb=(0, 0, 0), size=(0.5, 0.5, 0.5),
+                                                           max_nbr=256),
+                         [scenes.Obj(vtp.MAT_FLUID, pos)])
+    g, st = _compare_ugns(wl, walk=walk)
+    assert st["dead_particles"] == 0
+    assert g["neighbor_num"].max() < 256

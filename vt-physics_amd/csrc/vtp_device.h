@@ -69,9 +69,15 @@ __device__ __forceinline__ uint32_t cell_key(float3 p, uint32_t scene, const Gri
 
 // Squared distance with every operation rounded separately (no contraction):
 // ((dx*dx + dy*dy) + dz*dz), the evaluation order of dot() in the reference.
+// HIP's __fmul_rn / __fadd_rn are plain operators that the default
+// -ffp-contract=fast-honor-pragmas fuses into FMAs, so the operations are
+// written here under contract(off): the products stay rounded and pairs at
+// exactly the cut-off are decided as the FP32 restatement decides them.
 __device__ __forceinline__ float dist2_exact(float3 a, float3 b) {
-    float dx = __fsub_rn(a.x, b.x), dy = __fsub_rn(a.y, b.y), dz = __fsub_rn(a.z, b.z);
-    return __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+#pragma clang fp contract(off)
+    const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    const float xx = dx * dx, yy = dy * dy, zz = dz * dz;
+    return (xx + yy) + zz;
 }
 
 // Neighbour test of UGNSCudaImpl.cu:194: 0 < sqrtf(d2) <= cell.  sqrtf is
