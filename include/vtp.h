@@ -313,7 +313,10 @@ vtp_status vtp_load_state(vtp_ctx *ctx, const void *host_src, size_t bytes);
  * "static_lists" (PBF: 0 = default, a tick's search builds the lists of fluid
  * particles only, since no PBF kernel reads a static particle's list after
  * initialize; 1 = every list, as the reference, which vtp_debug_neighbors
- * needs after a tick). */
+ * needs after a tick),
+ * "radix_sort" (0 = default, the search groups particles by cell with a
+ * counting sort; 1 = a stable LSD radix sort of the cell keys.  Both give
+ * the same (cellId, attach id) order and bit-identical results). */
 vtp_status vtp_set_option(vtp_ctx *ctx, const char *key, double value);
 
 /* Per-stage device time (ms, summed over launches) and launch counts,

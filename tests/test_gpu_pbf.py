@@ -110,6 +110,30 @@ def test_pbf_repeat_runs_bit_identical():
     np.testing.assert_array_equal(a.read("vel"), b.read("vel"))
 
 
+def test_pbf_counting_and_radix_sort_bit_identical():
+    """The counting sort (default) and the stable radix sort give the same
+    (cellId, attach id) slot order, so every tick is bit-identical, including
+    particles that leave the grid, travel off it and fall back in."""
+    wl = scenes.cfg0_pbf(n_side=(16, 16, 16))
+    rng = np.random.default_rng(23)
+    jet = rng.uniform(0.3, 0.7, size=(500, 3)).astype(np.float32)
+    jet[:, 1] = rng.uniform(0.9, 1.05, 500)        # some start above the grid
+    vel = np.zeros_like(jet)
+    vel[:, 1] = rng.uniform(1.0, 3.0, 500)          # thrown up, out of the grid and back
+    wl.objects.append(scenes.Obj(vtp.MAT_FLUID, jet, vel))
+    a, _ = _pair(wl, graph=1)
+    b, _ = _pair(wl, graph=1, radix_sort=1)
+    dead = []
+    for _ in range(6):
+        a.step(10)
+        b.step(10)
+        np.testing.assert_array_equal(a.read("pos"), b.read("pos"))
+        np.testing.assert_array_equal(a.read("vel"), b.read("vel"))
+        assert a.stats()["dead_particles"] == b.stats()["dead_particles"]
+        dead.append(a.stats()["dead_particles"])
+    assert max(dead) > 0 and len(set(dead)) > 1
+
+
 def test_pbf_autotune_keeps_results_bit_identical():
     """The autotuner only changes workgroup sizes (per-particle results do not
     depend on them) and restores the state it timed from."""

@@ -10,11 +10,13 @@ from vtp import scenes
 pytestmark = pytest.mark.gpu
 
 
-def _compare_ugns(wl, params=None, walk=None):
+def _compare_ugns(wl, params=None, walk=None, radix=None):
     params = params or vtp.params_from_json(wl.solver, wl.config_json())
     ctx = vtp.Context(params)
     if walk is not None:
         ctx.set_option("neighbor_walk", walk)
+    if radix is not None:
+        ctx.set_option("radix_sort", radix)
     for o in wl.objects:
         ctx.add_particles(o.pos, o.mat, o.vel, o.vol_frac, o.scene)
     ctx.initialize()
@@ -41,13 +43,13 @@ def test_ugns_cfg0_positions_bit_exact():
     assert g["neighbor_num"].max() <= 60
 
 
-@pytest.mark.parametrize("walk", [1, 2])
-def test_ugns_random_points_bit_exact(walk):
+@pytest.mark.parametrize("walk,radix", [(1, 0), (2, 0), (1, 1)])
+def test_ugns_random_points_bit_exact(walk, radix):
     rng = np.random.default_rng(3)
     pos = rng.uniform(0.0, 0.3, size=(5000, 3)).astype(np.float32)
     wl = scenes.Workload("rand", "PBF", scenes.pbf_config(lb=(0, 0, 0), size=(0.3, 0.3, 0.3)),
                          [scenes.Obj(vtp.MAT_FLUID, pos)])
-    _compare_ugns(wl, walk=walk)
+    _compare_ugns(wl, walk=walk, radix=radix)
 
 
 @pytest.mark.parametrize("walk", [1, 2])
@@ -62,7 +64,8 @@ def test_ugns_dense_cap_truncation(walk):
     assert (g["neighbor_num"] == 20).all()
 
 
-def test_ugns_out_of_grid_and_duplicates():
+@pytest.mark.parametrize("radix", [0, 1])
+def test_ugns_out_of_grid_and_duplicates(radix):
     rng = np.random.default_rng(7)
     pos = rng.uniform(0.0, 0.2, size=(2000, 3)).astype(np.float32)
     pos[10] = [5.0, 0.1, 0.1]          # outside the grid
@@ -70,15 +73,16 @@ def test_ugns_out_of_grid_and_duplicates():
     pos[20] = pos[21]                  # coincident pair: distance 0 is not a neighbour
     wl = scenes.Workload("edge", "PBF", scenes.pbf_config(lb=(0, 0, 0), size=(0.2, 0.2, 0.2)),
                          [scenes.Obj(vtp.MAT_FLUID, pos)])
-    g, st = _compare_ugns(wl)
+    g, st = _compare_ugns(wl, radix=radix)
     assert st["dead_particles"] == 2
     assert g["neighbor_num"][10] == 0 and g["neighbor_num"][11] == 0
     assert 21 not in set(g["neighbors"][20, : g["neighbor_num"][20]])
 
 
-def test_ugns_batched_scenes_do_not_mix():
+@pytest.mark.parametrize("radix", [0, 1])
+def test_ugns_batched_scenes_do_not_mix(radix):
     wl = scenes.cfg3_batched(scenes=4, lo=2000, hi=9000)
-    g, _ = _compare_ugns(wl)
+    g, _ = _compare_ugns(wl, radix=radix)
     sc = np.concatenate([np.full(o.pos.shape[0], o.scene) for o in wl.objects])
     lists = g["neighbors"]
     for p in range(0, lists.shape[0], 97):

@@ -633,6 +633,9 @@ extern "C" vtp_status vtp_set_option(vtp_ctx *ctx, const char *key, double value
         } else if (k == "static_lists") {
             c.opt_static_lists = value != 0.0;
             pbf_capture_reset(c);
+        } else if (k == "radix_sort") {
+            c.ugns.radix = value != 0.0;
+            pbf_capture_reset(c);
         } else if (k == "neighbor_walk") {
             if (value != 0.0 && value != 1.0 && value != 2.0)
                 throw std::runtime_error("neighbor_walk must be 0 (adaptive), 1 (row) or 2 (reference)");

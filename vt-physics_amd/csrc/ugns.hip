@@ -184,6 +184,188 @@ __global__ __launch_bounds__(kRsThreads) void k_radix_scatter(const uint32_t *__
     }
 }
 
+// ---------------------------------------------------------------- counting sort
+// The in-cell order is fixed afterwards by k_cell_fixup (attach id), so the
+// grouping by cell needs no stable order for live particles: every particle
+// takes a rank in its cell from one atomic per run of equal keys, a scan over
+// the cells gives cellStart / cellEnd, and a scatter places the particle at
+// start + rank.  Particles off the grid keep the previous slot order (as the
+// stable radix sort does): their ranks come from per-tile counts.
+constexpr uint32_t kCsThreads = 256;         // one particle per thread, tile = workgroup
+constexpr uint32_t kScanPer = 4;             // cells per thread in the cell scan
+constexpr uint32_t kScanBlock = kCsThreads * kScanPer;
+
+__device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
+
+// Exclusive prefix of the per-wave values v[w] (w < 4) for wave w, via LDS.
+__device__ __forceinline__ uint32_t wave_offset(uint32_t *s4, int w) {
+    uint32_t o = 0;
+    for (int q = 0; q < w; ++q) o += s4[q];
+    return o;
+}
+
+__global__ __launch_bounds__(kCsThreads) void k_cell_count(const uint32_t *__restrict__ key, uint32_t n,
+                                                           uint32_t dead_key, uint32_t *__restrict__ count,
+                                                           uint32_t *__restrict__ rank,
+                                                           uint32_t *__restrict__ tile_dead) {
+    __shared__ uint32_t s_dead[kCsThreads / 64];
+    const uint32_t i = blockIdx.x * kCsThreads + threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool valid = i < n;
+    const uint32_t k = valid ? key[i] : kEmpty;
+    const bool live = valid && k < dead_key;
+    // runs of equal live keys in consecutive lanes (slots are cell-sorted
+    // from the previous search, so a cell is mostly one run)
+    const uint32_t kprev = __shfl_up(k, 1, 64);
+    const bool head = live && (lane == 0 || kprev != k);
+    const uint64_t heads = __ballot(head);
+    const uint64_t stops = heads | __ballot(!live);   // a run ends before the next head or non-live lane
+    const int h = live ? 63 - __clzll(heads & (~0ull >> (63 - lane))) : lane;
+    uint32_t base = 0;
+    if (head) {
+        const uint64_t above = stops & ~(~0ull >> (63 - lane));   // lanes > this one
+        const int next = above ? __ffsll((long long)above) - 1 : 64;
+        base = atomicAdd(&count[k], (uint32_t)(next - lane));
+    }
+    base = __shfl(base, h, 64);
+    if (live) rank[i] = base + (uint32_t)(lane - h);
+    const uint64_t dm = __ballot(valid && !live);
+    if (lane == 0) s_dead[w] = (uint32_t)__popcll(dm);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int q = 0; q < kCsThreads / 64; ++q) t += s_dead[q];
+        tile_dead[blockIdx.x] = t;
+    }
+}
+
+// Sum of kScanBlock cell counts per workgroup.
+__global__ __launch_bounds__(kCsThreads) void k_cell_scan_reduce(const uint32_t *__restrict__ count, uint32_t ncell,
+                                                                 uint32_t *__restrict__ bsum) {
+    __shared__ uint32_t s4[kCsThreads / 64];
+    const uint32_t b0 = blockIdx.x * kScanBlock + threadIdx.x * kScanPer;
+    uint32_t t = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kScanPer; ++q) t += b0 + q < ncell ? count[b0 + q] : 0u;
+    t = wave_sum_u32(t);
+    if ((threadIdx.x & 63) == 0) s4[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) bsum[blockIdx.x] = s4[0] + s4[1] + s4[2] + s4[3];
+}
+
+// Exclusive scan of a[0..m) in place by one workgroup (serial chunk per
+// thread, LDS scan of the chunk sums); returns the total in *total.
+__device__ void workgroup_exclusive_scan(uint32_t *a, uint32_t m, uint32_t *s, uint32_t *total) {
+    const uint32_t tid = threadIdx.x, per = (m + kCsThreads - 1) / kCsThreads;
+    const uint32_t b = tid * per, e = min(b + per, m);
+    uint32_t sum = 0;
+    for (uint32_t t = b; t < e; ++t) sum += a[t];
+    s[tid] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < kCsThreads; off <<= 1) {
+        const uint32_t v = tid >= off ? s[tid - off] : 0u;
+        __syncthreads();
+        s[tid] += v;
+        __syncthreads();
+    }
+    uint32_t run = s[tid] - sum;
+    for (uint32_t t = b; t < e; ++t) {
+        const uint32_t c = a[t];
+        a[t] = run;
+        run += c;
+    }
+    if (tid == kCsThreads - 1) *total = s[tid];
+    __syncthreads();
+}
+
+// One workgroup: block offsets of the cell scan, tile offsets of the
+// off-grid particles, and the dead key's range (after every live particle).
+__global__ __launch_bounds__(kCsThreads) void k_cell_scan_spine(uint32_t *__restrict__ bsum, uint32_t nblocks,
+                                                                uint32_t *__restrict__ tile_dead, uint32_t tiles,
+                                                                uint32_t n, uint32_t dead_key,
+                                                                uint32_t *__restrict__ start,
+                                                                uint32_t *__restrict__ end,
+                                                                uint32_t *__restrict__ meta) {
+    __shared__ uint32_t s[kCsThreads];
+    __shared__ uint32_t tot[2];
+    workgroup_exclusive_scan(bsum, nblocks, s, &tot[0]);
+    workgroup_exclusive_scan(tile_dead, tiles, s, &tot[1]);
+    if (threadIdx.x == 0) {
+        const uint32_t dead = tot[1], live = n - dead;
+        meta[0] = live;
+        start[dead_key] = dead ? live : kEmpty;
+        end[dead_key] = dead ? n - 1 : kEmpty;
+    }
+}
+
+// cellStart / cellEnd (inclusive, kEmpty for an empty cell) from the counts;
+// the counts are cleared for the next search.
+__global__ __launch_bounds__(kCsThreads) void k_cell_scan_ranges(uint32_t *__restrict__ count, uint32_t ncell,
+                                                                 const uint32_t *__restrict__ bsum,
+                                                                 uint32_t *__restrict__ start,
+                                                                 uint32_t *__restrict__ end) {
+    __shared__ uint32_t s4[kCsThreads / 64];
+    const uint32_t b0 = blockIdx.x * kScanBlock + threadIdx.x * kScanPer;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t c[kScanPer], t = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kScanPer; ++q) {
+        c[q] = b0 + q < ncell ? count[b0 + q] : 0u;
+        t += c[q];
+    }
+    // exclusive prefix of t over the workgroup: in-wave scan, then wave offsets
+    uint32_t incl = t;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+    }
+    if (lane == 63) s4[w] = incl;
+    __syncthreads();
+    uint32_t run = bsum[blockIdx.x] + wave_offset(s4, w) + incl - t;
+#pragma unroll
+    for (uint32_t q = 0; q < kScanPer; ++q) {
+        if (b0 + q < ncell) {
+            start[b0 + q] = c[q] ? run : kEmpty;
+            end[b0 + q] = c[q] ? run + c[q] - 1 : kEmpty;
+            if (c[q]) count[b0 + q] = 0u;
+        }
+        run += c[q];
+    }
+}
+
+// Places every particle: live ones at start[key] + rank, off-grid ones after
+// the live particles in previous-slot order.  Also writes the sorted key, the
+// old slot and its attach id.
+__global__ __launch_bounds__(kCsThreads) void k_cell_scatter(const uint32_t *__restrict__ key, uint32_t n,
+                                                             uint32_t dead_key, const uint32_t *__restrict__ rank,
+                                                             const uint32_t *__restrict__ start,
+                                                             const uint32_t *__restrict__ tile_dead,
+                                                             const uint32_t *__restrict__ meta,
+                                                             const uint32_t *__restrict__ orig_old,
+                                                             uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                                             uint32_t *__restrict__ so) {
+    __shared__ uint32_t s_dead[kCsThreads / 64];
+    const uint32_t i = blockIdx.x * kCsThreads + threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool valid = i < n;
+    const uint32_t k = valid ? key[i] : kEmpty;
+    const bool dead = valid && k >= dead_key;
+    const uint64_t dm = __ballot(dead);
+    if (lane == 0) s_dead[w] = (uint32_t)__popcll(dm);
+    __syncthreads();
+    if (!valid) return;
+    uint32_t dst;
+    if (dead)
+        dst = meta[0] + tile_dead[blockIdx.x] + wave_offset(s_dead, w) + (uint32_t)__popcll(dm & lanes_below(lane));
+    else
+        dst = start[k] + rank[i];
+    if (dst >= n) return;   // unreachable while the counts start at zero; never write past the arrays
+    kout[dst] = k;
+    vout[dst] = i;
+    so[dst] = orig_old[i];
+}
+
 // cellStart / cellEnd (inclusive) of every non-empty cell, UGNSCudaImpl.cu:144-166.
 __global__ void k_cell_ranges(const uint32_t *__restrict__ key, uint32_t n, uint32_t *__restrict__ start,
                               uint32_t *__restrict__ end) {
@@ -462,6 +644,14 @@ void Ugns::allocate(uint32_t n_, const Grid &g_, uint32_t max_neighbors) {
     }
     spine.alloc((size_t)256 * tiles);
     dtot.alloc(256);
+    // counting sort: per-cell counts (zero between searches), ranks, block
+    // and tile sums
+    count.alloc((size_t)g.dead_key);
+    VTP_HIP(hipMemset(count.p, 0, count.bytes()));
+    rank.alloc(n);
+    bsum.alloc(blocks_for(g.dead_key, kScanBlock));
+    tile_dead.alloc(blocks_for(n, kCsThreads));
+    meta.alloc(4);
     start.alloc((size_t)g.dead_key + 1);
     end.alloc((size_t)g.dead_key + 1);
     so.alloc(n);
@@ -475,6 +665,27 @@ void Ugns::allocate(uint32_t n_, const Grid &g_, uint32_t max_neighbors) {
 // Output: sorted keys in sorted_key(), permutation perm (old slot of each new
 // slot) and orig_new (attach id of each new slot).
 void Ugns::sort_and_rank(const uint32_t *orig_old, hipStream_t s, StageTimer &tm) {
+    if (!radix) {
+        const uint32_t ncell = g.dead_key;   // live keys; the dead key's range is set by the spine
+        const uint32_t ntile = blocks_for(n, kCsThreads), nblk = blocks_for(ncell, kScanBlock);
+        tm.begin(s, "ugns_sort");
+        hipLaunchKernelGGL(k_cell_count, dim3(ntile), dim3(kCsThreads), 0, s, key[0].p, n, g.dead_key, count.p,
+                           rank.p, tile_dead.p);
+        hipLaunchKernelGGL(k_cell_scan_reduce, dim3(nblk), dim3(kCsThreads), 0, s, count.p, ncell, bsum.p);
+        hipLaunchKernelGGL(k_cell_scan_spine, dim3(1), dim3(kCsThreads), 0, s, bsum.p, nblk, tile_dead.p, ntile, n,
+                           g.dead_key, start.p, end.p, meta.p);
+        hipLaunchKernelGGL(k_cell_scan_ranges, dim3(nblk), dim3(kCsThreads), 0, s, count.p, ncell, bsum.p, start.p,
+                           end.p);
+        hipLaunchKernelGGL(k_cell_scatter, dim3(ntile), dim3(kCsThreads), 0, s, key[0].p, n, g.dead_key, rank.p,
+                           start.p, tile_dead.p, meta.p, orig_old, key[1].p, val[1].p, so.p);
+        sorted = 1;
+        tm.end(s);
+        tm.begin(s, "ugns_ranges");
+        hipLaunchKernelGGL(k_cell_fixup, dim3(blocks_for(n, 256)), dim3(256), 0, s, key[1].p, val[1].p, so.p,
+                           start.p, end.p, n, g.dead_key, perm.p, orig_new.p);
+        tm.end(s);
+        return;
+    }
     tm.begin(s, "ugns_sort");
     int b = 0;
     for (uint32_t p = 0; p < passes; ++p) {

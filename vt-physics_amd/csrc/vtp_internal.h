@@ -111,7 +111,9 @@ struct Ugns {
     uint32_t n = 0, M = 0, cap = 0, stride = 0;
     uint32_t key_bits = 0, passes = 0, tiles = 0;
     int sorted = 0;                 // which key/val buffer holds the sorted result
-    DevBuf<uint32_t> key[2], val[2], spine, dtot;
+    DevBuf<uint32_t> key[2], val[2], spine, dtot;      // LSD radix sort (radix = true)
+    DevBuf<uint32_t> count, rank, bsum, tile_dead, meta; // counting sort by cell (default)
+    bool radix = false;
     DevBuf<uint32_t> start, end;    // [dead_key + 1], inclusive end, kEmpty = empty
     DevBuf<uint32_t> so, perm, orig_new;
     DevBuf<uint32_t> nbr, cnt;      // ELL lists nbr[t*stride + slot], count word per slot
