@@ -253,45 +253,53 @@ __global__ __launch_bounds__(kCsThreads) void k_cell_scan_reduce(const uint32_t 
     if (threadIdx.x == 0) bsum[blockIdx.x] = s4[0] + s4[1] + s4[2] + s4[3];
 }
 
-// Exclusive scan of a[0..m) in place by one workgroup (serial chunk per
-// thread, LDS scan of the chunk sums); returns the total in *total.
-__device__ void workgroup_exclusive_scan(uint32_t *a, uint32_t m, uint32_t *s, uint32_t *total) {
-    const uint32_t tid = threadIdx.x, per = (m + kCsThreads - 1) / kCsThreads;
-    const uint32_t b = tid * per, e = min(b + per, m);
-    uint32_t sum = 0;
-    for (uint32_t t = b; t < e; ++t) sum += a[t];
-    s[tid] = sum;
-    __syncthreads();
-    for (uint32_t off = 1; off < kCsThreads; off <<= 1) {
-        const uint32_t v = tid >= off ? s[tid - off] : 0u;
+// Exclusive scan of a[0..m) in place by one workgroup of kSpineThreads, in
+// rounds of kSpineThreads elements (wave scans, then the wave totals);
+// returns the total.
+constexpr uint32_t kSpineThreads = 1024;
+__device__ uint32_t workgroup_exclusive_scan(uint32_t *a, uint32_t m, uint32_t *s_w) {
+    const uint32_t tid = threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < m; base += kSpineThreads) {
+        const uint32_t idx = base + tid;
+        const uint32_t v = idx < m ? a[idx] : 0u;
+        uint32_t incl = v;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t u = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += u;
+        }
+        if (lane == 63) s_w[w] = incl;
         __syncthreads();
-        s[tid] += v;
-        __syncthreads();
+        uint32_t woff = 0, rtot = 0;
+#pragma unroll
+        for (int q = 0; q < (int)(kSpineThreads / 64); ++q) {
+            const uint32_t t = s_w[q];
+            woff += q < w ? t : 0u;
+            rtot += t;
+        }
+        if (idx < m) a[idx] = carry + woff + incl - v;
+        carry += rtot;
+        __syncthreads();   // s_w is rewritten by the next round
     }
-    uint32_t run = s[tid] - sum;
-    for (uint32_t t = b; t < e; ++t) {
-        const uint32_t c = a[t];
-        a[t] = run;
-        run += c;
-    }
-    if (tid == kCsThreads - 1) *total = s[tid];
-    __syncthreads();
+    return carry;
 }
 
 // One workgroup: block offsets of the cell scan, tile offsets of the
-// off-grid particles, and the dead key's range (after every live particle).
-__global__ __launch_bounds__(kCsThreads) void k_cell_scan_spine(uint32_t *__restrict__ bsum, uint32_t nblocks,
-                                                                uint32_t *__restrict__ tile_dead, uint32_t tiles,
-                                                                uint32_t n, uint32_t dead_key,
-                                                                uint32_t *__restrict__ start,
-                                                                uint32_t *__restrict__ end,
-                                                                uint32_t *__restrict__ meta) {
-    __shared__ uint32_t s[kCsThreads];
-    __shared__ uint32_t tot[2];
-    workgroup_exclusive_scan(bsum, nblocks, s, &tot[0]);
-    workgroup_exclusive_scan(tile_dead, tiles, s, &tot[1]);
+// off-grid particles (only when there are any), and the dead key's range
+// (after every live particle).
+__global__ __launch_bounds__(kSpineThreads) void k_cell_scan_spine(uint32_t *__restrict__ bsum, uint32_t nblocks,
+                                                                   uint32_t *__restrict__ tile_dead, uint32_t tiles,
+                                                                   uint32_t n, uint32_t dead_key,
+                                                                   uint32_t *__restrict__ start,
+                                                                   uint32_t *__restrict__ end,
+                                                                   uint32_t *__restrict__ meta) {
+    __shared__ uint32_t s_w[kSpineThreads / 64];
+    const uint32_t live = workgroup_exclusive_scan(bsum, nblocks, s_w);
+    const uint32_t dead = n - live;
+    if (dead) workgroup_exclusive_scan(tile_dead, tiles, s_w);
     if (threadIdx.x == 0) {
-        const uint32_t dead = tot[1], live = n - dead;
         meta[0] = live;
         start[dead_key] = dead ? live : kEmpty;
         end[dead_key] = dead ? n - 1 : kEmpty;
@@ -672,7 +680,7 @@ void Ugns::sort_and_rank(const uint32_t *orig_old, hipStream_t s, StageTimer &tm
         hipLaunchKernelGGL(k_cell_count, dim3(ntile), dim3(kCsThreads), 0, s, key[0].p, n, g.dead_key, count.p,
                            rank.p, tile_dead.p);
         hipLaunchKernelGGL(k_cell_scan_reduce, dim3(nblk), dim3(kCsThreads), 0, s, count.p, ncell, bsum.p);
-        hipLaunchKernelGGL(k_cell_scan_spine, dim3(1), dim3(kCsThreads), 0, s, bsum.p, nblk, tile_dead.p, ntile, n,
+        hipLaunchKernelGGL(k_cell_scan_spine, dim3(1), dim3(kSpineThreads), 0, s, bsum.p, nblk, tile_dead.p, ntile, n,
                            g.dead_key, start.p, end.p, meta.p);
         hipLaunchKernelGGL(k_cell_scan_ranges, dim3(nblk), dim3(kCsThreads), 0, s, count.p, ncell, bsum.p, start.p,
                            end.p);
