@@ -30,10 +30,10 @@ constexpr int kRsTile = kRsThreads * kRsRounds; // 1024 keys per workgroup (1.3k
 // kNbrStage words.
 constexpr uint32_t kNbrBuildThreads = 128;
 #ifndef VTP_NBR_STAGE
-#define VTP_NBR_STAGE 32
+#define VTP_NBR_STAGE 16
 #endif
 #ifndef VTP_BUILD_BATCH
-#define VTP_BUILD_BATCH 4
+#define VTP_BUILD_BATCH 8
 #endif
 #ifndef VTP_REF_BATCH
 #define VTP_REF_BATCH 8
@@ -41,7 +41,7 @@ constexpr uint32_t kNbrBuildThreads = 128;
 #ifndef VTP_REF_ORDER_SHARE
 #define VTP_REF_ORDER_SHARE 32
 #endif
-constexpr uint32_t kNbrStage = VTP_NBR_STAGE;   // list entries per thread staged in LDS (16 KB per workgroup)
+constexpr uint32_t kNbrStage = VTP_NBR_STAGE;   // list entries per thread staged in LDS (8 KB per workgroup)
 // A build walks every particle in the reference's cell order when more than
 // 1/kRefOrderShare of the particles reached the cap in the previous build.
 constexpr uint32_t kRefOrderShare = VTP_REF_ORDER_SHARE;
@@ -545,23 +545,37 @@ __global__ __launch_bounds__(kNbrBuildThreads) void k_build_neighbors(const floa
                 c = reference_scan(x, start, end, g, p, cx, cy, cz, sbase, cap, put);
             } else {
                 const int X0 = max(cx - 1, 0), X1 = min(cx + 1, g.gx - 1);
-                for (int oz = -1; oz <= 1; ++oz) {   // the 9 row ranges in key order: a sorted list
-                    const int Z = cz + oz;
-                    if (Z < 0 || Z >= g.gz) continue;
-                    for (int oy = -1; oy <= 1; ++oy) {
-                        const int Y = cy + oy;
-                        if (Y < 0 || Y >= g.gy) continue;
-                        const uint32_t row = sbase + (uint32_t)Z * gxy + (uint32_t)Y * (uint32_t)g.gx;
-                        uint32_t s = kEmpty, e = 0;
-                        for (int X = X0; X <= X1; ++X) {
-                            uint32_t a = ldg(start, row + X);
-                            if (a != kEmpty) {
-                                if (s == kEmpty) s = a;
-                                e = ldg(end, row + X);
-                            }
-                        }
-                        if (s != kEmpty && c <= cap) walk(s, e);   // past the cap: rebuilt below
+                // The 9 row ranges in key order (z outer, y inner): a sorted
+                // list.  The cell-table entries of row r + 1 are requested
+                // before row r is walked, and a row's starts and ends are
+                // loaded together (cells past X1 repeat X1), so the table
+                // loads cost no round trip of their own.
+                uint32_t ts[3], te[3];
+                auto request = [&](int r) {
+                    const int Z = cz + r / 3 - 1, Y = cy + r % 3 - 1;
+                    const bool ok = Z >= 0 && Z < g.gz && Y >= 0 && Y < g.gy;
+                    // clamped, so that the address is a cell of this scene even where !ok
+                    const uint32_t row = sbase + (uint32_t)min(max(Z, 0), g.gz - 1) * gxy +
+                                         (uint32_t)min(max(Y, 0), g.gy - 1) * (uint32_t)g.gx;
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        const uint32_t cell = row + (uint32_t)min(X0 + q, X1);
+                        ts[q] = ok ? ldg(start, cell) : kEmpty;
+                        te[q] = ok ? ldg(end, cell) : kEmpty;
                     }
+                };
+                request(0);
+                for (int r = 0; r < 9; ++r) {
+                    uint32_t s = kEmpty, e = 0;
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        if (ts[q] != kEmpty) {
+                            if (s == kEmpty) s = ts[q];
+                            e = te[q];
+                        }
+                    }
+                    if (r < 8) request(r + 1);
+                    if (s != kEmpty && c <= cap) walk(s, e);   // past the cap: rebuilt below
                 }
                 if (c > cap) c = reference_scan(x, start, end, g, p, cx, cy, cz, sbase, cap, put);
             }
