@@ -59,13 +59,18 @@ __global__ void k_adj_mat_from_orig(const uint32_t *__restrict__ orig, const int
 //   xvu[4i .. 4i+2] = (x_i, v_i, u_i)           for the XSPH adjoint
 //   xw [2i .. 2i+1] = (x_i, w_i)                for lambda_bar
 //   rec[4i .. 4i+3] = (x_i, (w_i, l_i), (G_i, rb_i), (sb_i, 0, 0, 0))  for xbar
+// Also writes the tick's per-slot materials (mat_attach of the taped slot map),
+// which every later reverse pass of the tick reads.
 __global__ void k_adj_u(const float4 *__restrict__ xb, const float4 *__restrict__ vb,
-                        const int32_t *__restrict__ mat, uint32_t n, float dt, const float4 *__restrict__ X,
-                        const float4 *__restrict__ V, float4 *__restrict__ xvu) {
+                        const uint32_t *__restrict__ orig, const int32_t *__restrict__ mat_attach, uint32_t n,
+                        float dt, const float4 *__restrict__ X, const float4 *__restrict__ V,
+                        float4 *__restrict__ xvu, int32_t *__restrict__ mat) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const int32_t m = mat_attach[orig[i]];
+    mat[i] = m;
     float4 a = make_float4(0, 0, 0, 0);
-    if (mat[i] == VTP_MAT_FLUID) {
+    if (m == VTP_MAT_FLUID) {
         float4 x = xb[i], v = vb[i];
         a = make_float4(x.x * dt + v.x, x.y * dt + v.y, x.z * dt + v.z, 0.0f);
     }
@@ -74,19 +79,15 @@ __global__ void k_adj_u(const float4 *__restrict__ xb, const float4 *__restrict_
     xvu[4 * (size_t)i + 2] = a;
 }
 
-// w = xbar + vbar/dt (adjoint of dx of a constraint iteration), zero for non-fluid.
-__global__ void k_adj_w(const float4 *__restrict__ xb, const float4 *__restrict__ vb,
-                        const int32_t *__restrict__ mat, uint32_t n, float inv_dt, const float4 *__restrict__ X,
-                        float4 *__restrict__ xw) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float4 a = make_float4(0, 0, 0, 0);
-    if (mat[i] == VTP_MAT_FLUID) {
-        float4 x = xb[i], v = vb[i];
-        a = make_float4(x.x + v.x * inv_dt, x.y + v.y * inv_dt, x.z + v.z * inv_dt, 0.0f);
-    }
-    xw[2 * (size_t)i] = X[i];
-    xw[2 * (size_t)i + 1] = a;
+// w = xbar + vbar/dt is the adjoint of dx of a constraint iteration (zero for
+// non-fluid).  The kernel that finishes xbar of a slot (XSPH adjoint for the
+// last iteration, xbar of iteration m + 1 for iteration m) writes the slot's
+// record xw = (x_m, w) in its epilogue.
+__device__ __forceinline__ void write_xw(float4 *__restrict__ xw, uint32_t p, const float4 &x, bool fluid,
+                                         const float4 &xb, const float4 &vb, float inv_dt) {
+    xw[2 * (size_t)p] = x;
+    xw[2 * (size_t)p + 1] = fluid ? make_float4(xb.x + vb.x * inv_dt, xb.y + vb.y * inv_dt, xb.z + vb.z * inv_dt, 0.0f)
+                                  : make_float4(0, 0, 0, 0);
 }
 
 // Transposed XSPH Jacobian:
@@ -100,9 +101,12 @@ template <uint32_t L>
 __global__ __launch_bounds__(256) void k_adj_xsph(const float4 *__restrict__ xvu, const int32_t *__restrict__ mat,
                                                   const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ cnt,
                                                   uint32_t n, uint32_t stride, PbfK q, float4 *__restrict__ xb,
-                                                  float4 *__restrict__ vb, double *kbar) {
+                                                  float4 *__restrict__ vb, double *kbar,
+                                                  const float4 *__restrict__ Xw, float4 *__restrict__ xw) {
     const uint32_t g = slot_index(), p = g / L, sub = g % L;
     const bool fluid = p < n && mat[p] == VTP_MAT_FLUID;
+    if (Xw && p < n && !fluid && sub == 0) write_xw(xw, p, Xw[p], false, make_float4(0, 0, 0, 0),
+                                                    make_float4(0, 0, 0, 0), q.inv_dt);
     double kp = 0.0;
     if (fluid) {
         const float4 xp = xvu[4 * (size_t)p], vp = xvu[4 * (size_t)p + 1], up = xvu[4 * (size_t)p + 2];
@@ -150,6 +154,7 @@ __global__ __launch_bounds__(256) void k_adj_xsph(const float4 *__restrict__ xvu
             xb[p] = a;
             vb[p] = b;
             kp = kk;
+            if (Xw) write_xw(xw, p, Xw[p], true, a, b, q.inv_dt);
         }
     }
     double s = wave_sum(kp);
@@ -207,9 +212,15 @@ __global__ __launch_bounds__(256) void k_adj_lambar(const float4 *__restrict__ x
 template <uint32_t L>
 __global__ __launch_bounds__(256) void k_adj_xbar(const float4 *__restrict__ rec, const int32_t *__restrict__ mat,
                                                   const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ cnt,
-                                                  uint32_t n, uint32_t stride, PbfK q, float4 *__restrict__ xb) {
+                                                  uint32_t n, uint32_t stride, PbfK q, float4 *__restrict__ xb,
+                                                  const float4 *__restrict__ vb, const float4 *__restrict__ Xw,
+                                                  float4 *__restrict__ xw) {
     const uint32_t g = slot_index(), p = g / L, sub = g % L;
-    if (p >= n || mat[p] != VTP_MAT_FLUID) return;
+    if (p >= n) return;
+    if (mat[p] != VTP_MAT_FLUID) {
+        if (Xw && sub == 0) write_xw(xw, p, Xw[p], false, make_float4(0, 0, 0, 0), make_float4(0, 0, 0, 0), q.inv_dt);
+        return;
+    }
     const float4 *const own = rec + 4 * (size_t)p;
     const float4 xp = own[0], wl = own[1], gr = own[2];
     const float4 wp = wl, Gp = gr;
@@ -250,6 +261,7 @@ __global__ __launch_bounds__(256) void k_adj_xbar(const float4 *__restrict__ rec
     a.y += ay;
     a.z += az;
     xb[p] = a;
+    if (Xw) write_xw(xw, p, Xw[p], true, a, vb[p], q.inv_dt);   // w of the previous iteration
 }
 
 // Undo the reorder (x_new[i] = x_old[perm[i]]) and the predict (v1 = v0 + g dt,
@@ -419,33 +431,32 @@ static void reverse_tick(Ctx &c, AdjointState &A, TapeBufs &t) {
     const uint32_t n = c.n, K = c.p.iteration_num, nb = div_up(n, 256);
     const PbfK q = pbf_constants(c);
     Ugns &u = c.ugns;
-    hipLaunchKernelGGL(k_adj_mat_from_orig, dim3(nb), dim3(256), 0, c.stream, t.orig.p, c.mat_attach.p,
-                       c.scene_attach.p, n, A.mat.p, (uint32_t *)nullptr);
+    // iteration m's record xw is written by the kernel that finishes xbar
+    // before it: the XSPH adjoint for m = K - 1, xbar of m + 1 otherwise
+    auto xw_of = [&](int m) -> const float4 * { return m >= 0 ? t.x[m].p : nullptr; };
     c.timer.begin(c.stream, "adj_xsph");
-    hipLaunchKernelGGL(k_adj_u, dim3(nb), dim3(256), 0, c.stream, A.xb.p, A.vb.p, A.mat.p, n, q.dt, t.x[K].p,
-                       t.vlast.p, A.xvu.p);
+    hipLaunchKernelGGL(k_adj_u, dim3(nb), dim3(256), 0, c.stream, A.xb.p, A.vb.p, t.orig.p, c.mat_attach.p, n, q.dt,
+                       t.x[K].p, t.vlast.p, A.xvu.p, A.mat.p);
     const uint32_t L = adj_lanes(c), nbl = div_up(n * L, 256);
     if (L == 4)
         hipLaunchKernelGGL(k_adj_xsph<4>, dim3(nbl), dim3(256), 0, c.stream, A.xvu.p, A.mat.p, t.nbr.p, t.cnt.p, n,
-                           u.stride, q, A.xb.p, A.vb.p, A.kbar.p);
+                           u.stride, q, A.xb.p, A.vb.p, A.kbar.p, xw_of((int)K - 1), A.xw.p);
     else
         hipLaunchKernelGGL(k_adj_xsph<1>, dim3(nbl), dim3(256), 0, c.stream, A.xvu.p, A.mat.p, t.nbr.p, t.cnt.p, n,
-                           u.stride, q, A.xb.p, A.vb.p, A.kbar.p);
+                           u.stride, q, A.xb.p, A.vb.p, A.kbar.p, xw_of((int)K - 1), A.xw.p);
     c.timer.end(c.stream);
     for (int m = (int)K - 1; m >= 0; --m) {
         c.timer.begin(c.stream, "adj_iteration");
-        hipLaunchKernelGGL(k_adj_w, dim3(nb), dim3(256), 0, c.stream, A.xb.p, A.vb.p, A.mat.p, n, q.inv_dt,
-                           t.x[m].p, A.xw.p);
         if (L == 4) {
             hipLaunchKernelGGL(k_adj_lambar<4>, dim3(nbl), dim3(256), 0, c.stream, A.xw.p, t.C[m].p, t.sg[m].p,
                                t.lam[m].p, A.mat.p, t.nbr.p, t.cnt.p, n, u.stride, q, A.rec.p);
             hipLaunchKernelGGL(k_adj_xbar<4>, dim3(nbl), dim3(256), 0, c.stream, A.rec.p, A.mat.p, t.nbr.p, t.cnt.p,
-                               n, u.stride, q, A.xb.p);
+                               n, u.stride, q, A.xb.p, A.vb.p, xw_of(m - 1), A.xw.p);
         } else {
             hipLaunchKernelGGL(k_adj_lambar<1>, dim3(nbl), dim3(256), 0, c.stream, A.xw.p, t.C[m].p, t.sg[m].p,
                                t.lam[m].p, A.mat.p, t.nbr.p, t.cnt.p, n, u.stride, q, A.rec.p);
             hipLaunchKernelGGL(k_adj_xbar<1>, dim3(nbl), dim3(256), 0, c.stream, A.rec.p, A.mat.p, t.nbr.p, t.cnt.p,
-                               n, u.stride, q, A.xb.p);
+                               n, u.stride, q, A.xb.p, A.vb.p, xw_of(m - 1), A.xw.p);
         }
         c.timer.end(c.stream);
     }
