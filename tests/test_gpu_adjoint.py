@@ -207,3 +207,38 @@ def test_backward_graph_replay_matches_eager():
     assert not np.array_equal(changed[0], first[0])
     assert np.array_equal(changed[0], changed_eager[0]) and changed[2] == changed_eager[2]
     ctx.close()
+
+
+def test_taped_last_segment_and_repeated_backward():
+    # The forward tapes the ticks of its last checkpoint segment, so the first
+    # backward reverses that segment without re-forwarding it; a second
+    # backward of the same rollout must re-forward it (the first overwrote the
+    # tapes).  A single-segment rollout is taped whole.  All give the same
+    # gradients bit for bit, and the lists handed back after the taped last
+    # tick give the same density loss as plain ticks.
+    wl = scene()
+    p, ctx = setup(wl)
+    cx, cv = loss_weights(wl)
+    x0, v0 = ctx.read("pos"), ctx.read("vel")
+
+    def restart():
+        ctx.write("pos", x0)
+        ctx.write("vel", v0)
+
+    restart()
+    ctx.step(STEPS)
+    loss_step, grad_step = ctx.density_loss()
+    restart()
+    ctx.rollout_fwd(STEPS, CKPT)
+    loss_fwd, grad_fwd = ctx.density_loss()
+    # the FP64 loss sum takes one atomic add per wavefront (order not fixed)
+    assert abs(loss_fwd - loss_step) <= 1e-12 * abs(loss_step) and np.array_equal(grad_fwd, grad_step)
+    first = ctx.rollout_bwd(cx, cv)
+    second = ctx.rollout_bwd(cx, cv)
+    restart()
+    ctx.rollout_fwd(STEPS, STEPS)
+    whole = ctx.rollout_bwd(cx, cv)
+    whole_again = ctx.rollout_bwd(cx, cv)
+    for r in (second, whole, whole_again):
+        assert np.array_equal(first[0], r[0]) and np.array_equal(first[1], r[1]) and first[2] == r[2]
+    ctx.close()

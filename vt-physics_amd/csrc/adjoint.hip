@@ -353,13 +353,20 @@ struct AdjointState {
     // One captured graph per checkpoint segment (reload, taped re-forward,
     // reverse), replayed by every later backward of the same rollout shape.
     std::vector<hipGraphExec_t> seg_exec;
+    hipGraphExec_t tail_exec = nullptr;   // the last segment's reverse alone (its ticks taped by the forward)
     uint64_t seg_epoch = ~0ull;
     uint32_t seg_nsteps = 0, seg_every = 0;
     float4 *canon[4] = {};   // xb, vb, xb2, vb2 as allocated (the reverse ping-pongs them)
+    // The forward tapes the ticks of its last segment, so the first segment a
+    // backward reverses needs no re-forward.  A backward that re-forwards an
+    // earlier segment overwrites those tapes.
+    bool tail_taped = false;
     void drop_graphs() {
         for (auto &e : seg_exec)
             if (e) (void)hipGraphExecDestroy(e);
         seg_exec.clear();
+        if (tail_exec) (void)hipGraphExecDestroy(tail_exec);
+        tail_exec = nullptr;
     }
     ~AdjointState() { drop_graphs(); }
     DevBuf<float4> xb, vb, xb2, vb2;
@@ -421,6 +428,33 @@ static void alloc_tape(Ctx &c, TapeBufs &t) {
         t.ptrs.sg[m] = t.sg[m].p;
     }
     t.ptrs.v_last = t.vlast.p;
+}
+
+// Tapes for one checkpoint segment and the adjoint buffers, allocated once per
+// context (and again when the segment grows).
+static void ensure_tapes(Ctx &c, AdjointState &A) {
+    if (A.tape.size() >= A.every) return;
+    const uint32_t n = c.n;
+    A.tape.clear();
+    for (uint32_t t = 0; t < A.every; ++t) {
+        auto tb = std::make_unique<TapeBufs>();
+        alloc_tape(c, *tb);
+        A.tape.push_back(std::move(tb));
+    }
+    A.xb.alloc(n);
+    A.vb.alloc(n);
+    A.xb2.alloc(n);
+    A.vb2.alloc(n);
+    A.xvu.alloc(4 * (size_t)n);
+    A.xw.alloc(2 * (size_t)n);
+    A.rec.alloc(4 * (size_t)n);
+    A.kbar.alloc(1);
+    A.mat.alloc(n);
+    A.canon[0] = A.xb.p;
+    A.canon[1] = A.vb.p;
+    A.canon[2] = A.xb2.p;
+    A.canon[3] = A.vb2.p;
+    A.drop_graphs();   // captured against the old tapes
 }
 
 // Lanes per particle of the reverse passes: 4 on scenes too small to fill the
@@ -518,12 +552,24 @@ extern "C" vtp_status vtp_pbf_rollout_fwd(vtp_ctx *ctx, uint32_t nsteps, uint32_
                 A.ckpt.push_back(std::move(cp));
             }
         }
+        ensure_tapes(c, A);
+        A.tail_taped = false;
         save_ckpt(c, *A.ckpt[0]);
+        const uint32_t tail0 = (A.nseg - 1) * ckpt_every;   // first tick of the last segment
         for (uint32_t t = 0; t < nsteps; ++t) {
-            pbf_step(c);
+            if (t >= tail0) pbf_step_taped(c, A.tape[t - tail0]->ptrs);
+            else pbf_step(c);
             c.steps += 1;
             c.sim_time += c.p.time_step;
             if ((t + 1) % ckpt_every == 0 || t + 1 == nsteps) save_ckpt(c, *A.ckpt[(t + ckpt_every) / ckpt_every]);
+        }
+        {   // the last tick's search wrote into its tape: the context gets its lists back
+            const TapeBufs &last = *A.tape[nsteps - 1 - tail0];
+            Ugns &u = c.ugns;
+            VTP_HIP(hipMemcpyAsync(u.nbr.p, last.nbr.p, u.nbr.bytes(), hipMemcpyDeviceToDevice, c.stream));
+            VTP_HIP(hipMemcpyAsync(u.cnt.p, last.cnt.p, u.cnt.bytes(), hipMemcpyDeviceToDevice, c.stream));
+            VTP_HIP(hipMemcpyAsync(u.perm.p, last.perm.p, u.perm.bytes(), hipMemcpyDeviceToDevice, c.stream));
+            c.lists_stale = false;
         }
         read_status(c);
         if (c.h_status[kStCrash] & ~(uint32_t)VTP_CRASH_OUT_OF_GRID) {
@@ -532,6 +578,7 @@ extern "C" vtp_status vtp_pbf_rollout_fwd(vtp_ctx *ctx, uint32_t nsteps, uint32_
             return VTP_ECRASH;
         }
         A.valid = true;
+        A.tail_taped = true;
         return VTP_OK;
     });
 }
@@ -546,28 +593,8 @@ extern "C" vtp_status vtp_pbf_rollout_bwd(vtp_ctx *ctx, const float *grad_x, con
     if (!A.valid) { c.err = "vtp_pbf_rollout_bwd needs a preceding vtp_pbf_rollout_fwd"; return VTP_ESTATE; }
     return adj_guarded(c, [&]() -> vtp_status {
         const uint32_t n = c.n;
-        if (A.tape.size() < A.every) {
-            A.tape.clear();
-            for (uint32_t t = 0; t < A.every; ++t) {
-                auto tb = std::make_unique<TapeBufs>();
-                alloc_tape(c, *tb);
-                A.tape.push_back(std::move(tb));
-            }
-            A.xb.alloc(n);
-            A.vb.alloc(n);
-            A.xb2.alloc(n);
-            A.vb2.alloc(n);
-            A.xvu.alloc(4 * (size_t)n);
-            A.xw.alloc(2 * (size_t)n);
-            A.rec.alloc(4 * (size_t)n);
-            A.kbar.alloc(1);
-            A.mat.alloc(n);
-            A.canon[0] = A.xb.p;
-            A.canon[1] = A.vb.p;
-            A.canon[2] = A.xb2.p;
-            A.canon[3] = A.vb2.p;
-            A.drop_graphs();   // captured against the old tapes
-        }
+        if (A.tape.size() < A.every) A.tail_taped = false;   // (re)allocated: nothing taped
+        ensure_tapes(c, A);
         if (A.seg_epoch != c.capture_epoch || A.seg_nsteps != A.nsteps || A.seg_every != A.every ||
             A.seg_exec.size() != A.nseg) {
             A.drop_graphs();
@@ -601,14 +628,19 @@ extern "C" vtp_status vtp_pbf_rollout_bwd(vtp_ctx *ctx, const float *grad_x, con
         for (int seg = (int)A.nseg - 1; seg >= 0; --seg) {
             const uint32_t s0 = (uint32_t)seg * A.every;
             const uint32_t len = std::min(A.every, A.nsteps - s0);
+            const bool taped = A.tail_taped && seg == (int)A.nseg - 1;   // the forward's own tapes
             auto enqueue = [&]() {
-                load_ckpt(c, *A.ckpt[seg]);
-                for (uint32_t t = 0; t < len; ++t) pbf_step_taped(c, A.tape[t]->ptrs);
+                if (!taped) {
+                    load_ckpt(c, *A.ckpt[seg]);
+                    for (uint32_t t = 0; t < len; ++t) pbf_step_taped(c, A.tape[t]->ptrs);
+                }
                 for (int t = (int)len - 1; t >= 0; --t) reverse_tick(c, A, *A.tape[t]);
             };
+            if (!taped) A.tail_taped = false;   // this segment's re-forward overwrites the tapes
+            hipGraphExec_t &exec = taped ? A.tail_exec : A.seg_exec[seg];
             if (!graphs) {
                 enqueue();
-            } else if (!A.seg_exec[seg]) {
+            } else if (!exec) {
                 VTP_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
                 hipGraph_t g = nullptr;
                 try {
@@ -619,12 +651,12 @@ extern "C" vtp_status vtp_pbf_rollout_bwd(vtp_ctx *ctx, const float *grad_x, con
                     throw;
                 }
                 VTP_HIP(hipStreamEndCapture(c.stream, &g));
-                hipError_t e = hipGraphInstantiate(&A.seg_exec[seg], g, nullptr, nullptr, 0);
+                hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(g);
                 VTP_HIP(e);
-                VTP_HIP(hipGraphLaunch(A.seg_exec[seg], c.stream));
+                VTP_HIP(hipGraphLaunch(exec, c.stream));
             } else {
-                VTP_HIP(hipGraphLaunch(A.seg_exec[seg], c.stream));
+                VTP_HIP(hipGraphLaunch(exec, c.stream));
                 for (uint32_t t = 0; t < len; ++t) {   // the swaps the captured reverse ticks made
                     std::swap(A.xb.p, A.xb2.p);
                     std::swap(A.vb.p, A.vb2.p);
