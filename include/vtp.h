@@ -316,7 +316,10 @@ vtp_status vtp_load_state(vtp_ctx *ctx, const void *host_src, size_t bytes);
  * needs after a tick),
  * "radix_sort" (0 = default, the search groups particles by cell with a
  * counting sort; 1 = a stable LSD radix sort of the cell keys.  Both give
- * the same (cellId, attach id) order and bit-identical results). */
+ * the same (cellId, attach id) order and bit-identical results),
+ * "full_flux_retry" (IMM / IMM-CT: 0 = default, a negative-fraction retry
+ * recomputes the fluxes of the particles next to newly flagged ones; 1 =
+ * every retry recomputes all fluxes, as the reference.  Bit-identical). */
 vtp_status vtp_set_option(vtp_ctx *ctx, const char *key, double value);
 
 /* Per-stage device time (ms, summed over launches) and launch counts,

@@ -633,6 +633,8 @@ extern "C" vtp_status vtp_set_option(vtp_ctx *ctx, const char *key, double value
         } else if (k == "static_lists") {
             c.opt_static_lists = value != 0.0;
             pbf_capture_reset(c);
+        } else if (k == "full_flux_retry") {
+            c.opt_full_flux_retry = value != 0.0;
         } else if (k == "radix_sort") {
             c.ugns.radix = value != 0.0;
             pbf_capture_reset(c);

@@ -538,12 +538,35 @@ __global__ __launch_bounds__(256) void k_imm_fluxes(const float4 *__restrict__ x
                                                     const uint32_t *__restrict__ flag,
                                                     const uint32_t *__restrict__ nbr,
                                                     const uint32_t *__restrict__ cnt, uint32_t n, uint32_t stride,
-                                                    ImmK q, Ph ph, const uint32_t *status, const SolverScalars *sc) {
+                                                    ImmK q, Ph ph, const uint32_t *status, const SolverScalars *sc,
+                                                    int retry) {
     if (status[kStDone]) return;
     uint32_t i = slot_index();
     if (i >= n) return;
     const int32_t mi = mat[i];
     if (mi != VTP_MAT_FLUID) return;
+    if (retry) {
+        // A retry round changes the fluxes of a particle only if it or one of
+        // its neighbours was flagged by the check just before (tag = that
+        // round's iteration); every other particle keeps the fluxes of the
+        // previous round, which came from the same inputs.
+        const uint32_t tag = status[kStIter] - 1;
+        const uint32_t fi0 = flag[i];
+        if (fi0 != tag) {
+            if (fi0) return;   // flagged in an earlier round: its zero fluxes stand
+            const uint32_t nn = cnt[i] & ~kDeadBit;
+            const uint32_t *p = nbr + i;
+            bool dirty = false;
+            for (uint32_t t = 0; t < nn && !dirty; t += 8) {
+                uint32_t j[8];
+#pragma unroll
+                for (uint32_t b = 0; b < 8; ++b) j[b] = p[(size_t)min(t + b, nn - 1) * stride];
+#pragma unroll
+                for (uint32_t b = 0; b < 8; ++b) dirty |= flag[j[b]] == tag;
+            }
+            if (!dirty) return;
+        }
+    }
     float fin[MP], fout[MP], fi[MP];
     float4 di[MP];
 #pragma unroll
@@ -614,7 +637,7 @@ __global__ void k_imm_check_negative(const int32_t *__restrict__ mat, uint32_t *
             size_t o = (size_t)k * ph.stride + i;
             if (ph.frac[o] + ph.fout[o] + ph.fin[o] < 0.0f) neg = true;
         }
-        if (neg) flag[i] = 1;
+        if (neg) flag[i] = status[kStIter];   // nonzero: the round that flagged it
     }
     uint64_t b = __ballot(neg);
     if ((threadIdx.x & 63) == 0 && b) atomicAdd(&status[kStNegCount], (uint32_t)__popcll(b));
@@ -857,7 +880,7 @@ static void imm_step_impl(Ctx &c) {
     hipLaunchKernelGGL(k_phase_begin, dim3(1), dim3(64), 0, c.stream, c.status.p);
     c.timer.begin(c.stream, "imm_fluxes");
     hipLaunchKernelGGL(k_imm_fluxes<MP>, dim3(nbB), dim3(B), 0, c.stream, x, v, c.mat.p, c.flag.p, u.nbr.p, u.cnt.p,
-                       n, st, q, ph, c.status.p, c.scal.p);
+                       n, st, q, ph, c.status.p, c.scal.p, 0);
     c.timer.end(c.stream);
     uint32_t chunk = 2, launched = 0;
     for (;;) {
@@ -866,7 +889,7 @@ static void imm_step_impl(Ctx &c) {
                                ph, c.status.p);
             hipLaunchKernelGGL(k_imm_phase_decide, dim3(1), dim3(64), 0, c.stream, c.status.p, cap);
             hipLaunchKernelGGL(k_imm_fluxes<MP>, dim3(nbB), dim3(B), 0, c.stream, x, v, c.mat.p, c.flag.p, u.nbr.p,
-                               u.cnt.p, n, st, q, ph, c.status.p, c.scal.p);
+                               u.cnt.p, n, st, q, ph, c.status.p, c.scal.p, c.opt_full_flux_retry ? 0 : 1);
         }
         read_status(c);
         if (c.h_status[kStDone] || launched > cap) break;

@@ -231,6 +231,7 @@ struct Ctx {
     uint32_t opt_chunk = 0;   // 0 = adaptive
     uint32_t opt_lanes = 1;   // lanes per particle in the PBF neighbour passes (1 or 4)
     bool opt_static_lists = false;   // PBF ticks also build the (unread) lists of static particles
+    bool opt_full_flux_retry = false;  // IMM: every negative-fraction retry recomputes all fluxes
     StageTimer timer;
 
     // attach-order tables (static)
