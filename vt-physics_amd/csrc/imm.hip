@@ -400,11 +400,15 @@ __global__ __launch_bounds__(256) void k_immct_conformation(const float4 *__rest
                                                             uint32_t stride, ImmK q, const float *__restrict__ fpoly,
                                                             float *__restrict__ Q, float *__restrict__ Lout,
                                                             float *__restrict__ stress, float *__restrict__ vis,
+                                                            float4 *__restrict__ srec,
                                                             const SolverScalars *sc) {
     uint32_t i = slot_index();
     if (i >= n) return;
     if (mat[i] != VTP_MAT_FLUID) {
         vis[i] = q.vis_base;
+        srec[3 * (size_t)i] = make_float4(0, 0, 0, 0);
+        srec[3 * (size_t)i + 1] = make_float4(0, 0, 0, 0);
+        srec[3 * (size_t)i + 2] = make_float4(0, 0, 0, 0);   // y = 0: not fluid
         return;
     }
     const float dt = sc->dt;
@@ -457,12 +461,20 @@ __global__ __launch_bounds__(256) void k_immct_conformation(const float4 *__rest
     s5 = s5 * s5 * s5 * s5 * s5;
     mu = q.vis_base + (mu - q.vis_base) / (1.0f + s5) * fminf(1.0f, ap / (q.theta + 1e-6f));
     vis[i] = mu;
+    float S[9];
 #pragma unroll
     for (int a = 0; a < 9; ++a) {
+        S[a] = mu * (Qn[a] - ((a == 0 || a == 4 || a == 8) ? 1.0f : 0.0f));
         Q[a * st + i] = Qn[a];
         Lout[a * st + i] = L[a];
-        stress[a * st + i] = mu * (Qn[a] - ((a == 0 || a == 4 || a == 8) ? 1.0f : 0.0f));
+        stress[a * st + i] = S[a];
     }
+    // the polymer-stress pass gathers S / rest^2 of a neighbour as one 48-byte record
+    const float ri = rest[i];
+    const float ir2 = 1.0f / (ri * ri);
+    srec[3 * (size_t)i] = make_float4(S[0] * ir2, S[1] * ir2, S[2] * ir2, S[3] * ir2);
+    srec[3 * (size_t)i + 1] = make_float4(S[4] * ir2, S[5] * ir2, S[6] * ir2, S[7] * ir2);
+    srec[3 * (size_t)i + 2] = make_float4(S[8] * ir2, 1.0f, 0.0f, 0.0f);   // y = 1: fluid
 }
 
 // add_phase_pct_acc (IMMCTCudaImpl.cu:746-780) + phase velocities + mixture velocity:
@@ -475,7 +487,7 @@ __global__ __launch_bounds__(256) void k_immct_stress_acc(const float4 *__restri
                                                           const float *__restrict__ rest,
                                                           const float *__restrict__ cd,
                                                           const float *__restrict__ fpoly,
-                                                          const float *__restrict__ stress,
+                                                          const float4 *__restrict__ srec,
                                                           const uint32_t *__restrict__ nbr,
                                                           const uint32_t *__restrict__ cnt, uint32_t n,
                                                           uint32_t stride, ImmK q, Ph ph, const SolverScalars *sc) {
@@ -485,31 +497,34 @@ __global__ __launch_bounds__(256) void k_immct_stress_acc(const float4 *__restri
     const float dt = sc->dt;
     const float4 xi = x[i];
     const float ri = rest[i];
-    const float ir2 = 1.0f / (ri * ri);
-    const size_t st = stride;
     float Si[9];
-#pragma unroll
-    for (int a = 0; a < 9; ++a) Si[a] = stress[a * st + i] * ir2;
+    {
+        const float4 a = srec[3 * (size_t)i], b = srec[3 * (size_t)i + 1], c = srec[3 * (size_t)i + 2];
+        Si[0] = a.x; Si[1] = a.y; Si[2] = a.z; Si[3] = a.w;
+        Si[4] = b.x; Si[5] = b.y; Si[6] = b.z; Si[7] = b.w;
+        Si[8] = c.x;
+    }
     const uint32_t nn = cnt[i] & ~kDeadBit;
     float ax = 0, ay = 0, az = 0;
-    const uint32_t *p = nbr + i;
-    for (uint32_t t = 0; t < nn; ++t) {
-        uint32_t j = p[(size_t)t * stride];
-        if (mat[j] != VTP_MAT_FLUID) continue;
-        float4 xj = x[j];
-        float rj = rest[j];
-        float jr2 = 1.0f / (rj * rj);
-        float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        float f = cubic_grad_factor(dx * dx + dy * dy + dz * dz, q.k);
-        if (q.mass_weighted) f *= xj.w;
-        float gx = dx * f, gy = dy * f, gz = dz * f;
-        float T[9];
+    // a neighbour is (x_j, S_j / rest_j^2 with its fluid flag): 64 bytes
+    struct D { float4 x, a, b, c; };
+    for_each_neighbor(
+        nbr, stride, i, nn,
+        [&](uint32_t j) { return D{ldg(x, j), ldg(srec, 3 * j), ldg(srec, 3 * j + 1), ldg(srec, 3 * j + 2)}; },
+        [&](const D &d) {
+            if (d.c.y == 0.0f) return;   // not fluid
+            float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
+            float f = cubic_grad_factor(dx * dx + dy * dy + dz * dz, q.k);
+            if (q.mass_weighted) f *= d.x.w;
+            float gx = dx * f, gy = dy * f, gz = dz * f;
+            const float Sj[9] = {d.a.x, d.a.y, d.a.z, d.a.w, d.b.x, d.b.y, d.b.z, d.b.w, d.c.x};
+            float T[9];
 #pragma unroll
-        for (int a = 0; a < 9; ++a) T[a] = Si[a] + stress[a * st + j] * jr2;
-        ax += T[0] * gx + T[1] * gy + T[2] * gz;
-        ay += T[3] * gx + T[4] * gy + T[5] * gz;
-        az += T[6] * gx + T[7] * gy + T[8] * gz;
-    }
+            for (int a = 0; a < 9; ++a) T[a] = Si[a] + Sj[a];
+            ax += T[0] * gx + T[1] * gy + T[2] * gz;
+            ay += T[3] * gx + T[4] * gy + T[5] * gz;
+            az += T[6] * gx + T[7] * gy + T[8] * gz;
+        });
     const float sc_a = fpoly[i] * ri;
     ax *= sc_a;
     ay *= sc_a;
@@ -786,6 +801,7 @@ void imm_initialize(Ctx &c) {
     c.vis_ct.alloc(st);
     c.Lgrad.alloc(9 * (size_t)st);
     c.stress.alloc(9 * (size_t)st);
+    c.srec.alloc(3 * (size_t)st);
     if (c.p.solver != VTP_IMMCT) c.Q[0].alloc(9);
     VTP_HIP(hipMemsetAsync(c.flag.p, 0, c.flag.bytes(), c.stream));
     VTP_HIP(hipMemsetAsync(c.Lgrad.p, 0, c.Lgrad.bytes(), c.stream));
@@ -863,11 +879,11 @@ static void imm_step_impl(Ctx &c) {
     if (q.ct) {
         c.timer.begin(c.stream, "immct_conformation");
         hipLaunchKernelGGL(k_immct_conformation, dim3(nbB), dim3(B), 0, c.stream, x, v, c.mat.p, rest, u.nbr.p,
-                           u.cnt.p, n, st, q, c.fpoly.p, c.Q[0].p, c.Lgrad.p, c.stress.p, c.vis_ct.p, c.scal.p);
+                           u.cnt.p, n, st, q, c.fpoly.p, c.Q[0].p, c.Lgrad.p, c.stress.p, c.vis_ct.p, c.srec.p, c.scal.p);
         c.timer.end(c.stream);
         c.timer.begin(c.stream, "immct_stress_acc");
         hipLaunchKernelGGL(k_immct_stress_acc<MP>, dim3(nbB), dim3(B), 0, c.stream, x, v, c.vadv.p, c.mat.p, rest,
-                           c.cd.p, c.fpoly.p, c.stress.p, u.nbr.p, u.cnt.p, n, st, q, ph, c.scal.p);
+                           c.cd.p, c.fpoly.p, c.srec.p, u.nbr.p, u.cnt.p, n, st, q, ph, c.scal.p);
         c.timer.end(c.stream);
     }
     df_pressure_solve(c, x, v, 1);

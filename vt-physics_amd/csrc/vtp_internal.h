@@ -285,6 +285,7 @@ struct Ctx {
     DevBuf<float> rest2[2];      // IMM rest-density state (ping-pong)
     DevBuf<float> Q[2];          // 9 per particle, [c*stride + slot]
     DevBuf<float> Lgrad, stress; // 9 per particle
+    DevBuf<float4> srec;         // IMM-CT: S / rest^2 of a slot as three float4 (fluid flag in [2].y)
     DevBuf<float> vis_ct, rheo_beta, rheo_kappa;
 
     // per-iteration launch shapes picked by the autotuner
