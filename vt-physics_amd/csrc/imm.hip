@@ -91,7 +91,9 @@ __device__ __forceinline__ float4 mix_velocity(const Ph &ph, uint32_t i, int P, 
     for (int k = 0; k < MP; ++k) {
         if (k >= P) break;
         float4 u = ph.u[(size_t)k * ph.stride + i];
-        ph.drift[(size_t)k * ph.stride + i] = f4(u.x - vx, u.y - vy, u.z - vz);
+        // w: the fraction, read with the drift by the phase-flux pass
+        ph.drift[(size_t)k * ph.stride + i] = make_float4(u.x - vx, u.y - vy, u.z - vz,
+                                                          ph.frac[(size_t)k * ph.stride + i]);
     }
     float4 v = make_float4(vx, vy, vz, vold.w);
     vadv[i] = v;
@@ -595,30 +597,42 @@ __global__ __launch_bounds__(256) void k_imm_fluxes(const float4 *__restrict__ x
         const float dt = sc->dt;
         const float4 xi = x[i];
         const uint32_t nn = cnt[i] & ~kDeadBit;
-        const uint32_t *p = nbr + i;
-        for (uint32_t t = 0; t < nn; ++t) {
-            uint32_t j = p[(size_t)t * stride];
-            if (mat[j] != mi || flag[j]) continue;
-            float4 xj = x[j];
-            float Vj = v[j].w;
-            float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            float f = cubic_grad_factor(r2, q.k);
-            float gx = dx * f, gy = dy * f, gz = dz * f;
-            float factor = f;   // (gradW . x_ij) / |x_ij|^2 = f
+        // a neighbour: x, V, material, flag and per phase (drift, fraction);
+        // mix_velocity keeps the fraction in the drift's w
+        struct D { float4 x; float V; int32_t m; uint32_t f; float4 pd[MP]; };
+        for_each_neighbor(
+            nbr, stride, i, nn,
+            [&](uint32_t j) {
+                D d;
+                d.x = ldg(x, j);
+                d.V = ldg_w(v, j);
+                d.m = ldg(mat, j);
+                d.f = ldg(flag, j);
 #pragma unroll
-            for (int k = 0; k < MP; ++k) {
-                if (k >= q.P) break;
-                size_t o = (size_t)k * ph.stride + j;
-                float fj = ph.frac[o];
-                float4 dj = ph.drift[o];
-                float c1 = -dt * Vj * ((fi[k] * di[k].x + fj * dj.x) * gx + (fi[k] * di[k].y + fj * dj.y) * gy +
-                                       (fi[k] * di[k].z + fj * dj.z) * gz);
-                if (c1 < 0.0f) fout[k] += c1; else fin[k] += c1;
-                float c2 = dt * q.cf * (fi[k] - fj) * Vj * factor;
-                if (c2 < 0.0f) fout[k] += c2; else fin[k] += c2;
-            }
-        }
+                for (int k = 0; k < MP; ++k)
+                    d.pd[k] = k < q.P ? ldg(ph.drift + (size_t)k * ph.stride, j) : make_float4(0, 0, 0, 0);
+                return d;
+            },
+            [&](const D &d) {
+                if (d.m != mi || d.f) return;
+                const float Vj = d.V;
+                float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
+                float r2 = dx * dx + dy * dy + dz * dz;
+                float f = cubic_grad_factor(r2, q.k);
+                float gx = dx * f, gy = dy * f, gz = dz * f;
+                float factor = f;   // (gradW . x_ij) / |x_ij|^2 = f
+#pragma unroll
+                for (int k = 0; k < MP; ++k) {
+                    if (k >= q.P) break;
+                    const float fj = d.pd[k].w;
+                    const float4 dj = d.pd[k];
+                    float c1 = -dt * Vj * ((fi[k] * di[k].x + fj * dj.x) * gx + (fi[k] * di[k].y + fj * dj.y) * gy +
+                                           (fi[k] * di[k].z + fj * dj.z) * gz);
+                    if (c1 < 0.0f) fout[k] += c1; else fin[k] += c1;
+                    float c2 = dt * q.cf * (fi[k] - fj) * Vj * factor;
+                    if (c2 < 0.0f) fout[k] += c2; else fin[k] += c2;
+                }
+            });
     }
 #pragma unroll
     for (int k = 0; k < MP; ++k) {
