@@ -6,10 +6,13 @@
 // Differences in structure (results are identical):
 //  * particle state is kept in cell-sorted slot order and physically
 //    reordered after every sort, so neighbour gathers hit contiguous memory;
-//  * the sort is a stable LSD radix sort written here: 8-bit digits, a
-//    per-tile digit histogram, a per-digit spine scan and a scatter whose
-//    stable in-tile ranks come from wavefront ballots (one 64-bit ballot per
-//    key bit gives every lane the mask of lanes holding the same digit);
+//  * particles are grouped by cell with a counting sort written here (one
+//    atomic per run of equal keys in a wavefront, a scan over the cells that
+//    yields cellStart/cellEnd directly, a scatter); a stable LSD radix sort
+//    (8-bit digits, a per-tile digit histogram, a per-digit spine scan and a
+//    scatter whose stable in-tile ranks come from wavefront ballots: one
+//    64-bit ballot per key bit gives every lane the mask of lanes holding the
+//    same digit) is kept as an option and gives the same order;
 //  * because the input is the previous step's slot order rather than attach
 //    order, equal cell ids are put in attach-id order by a per-cell rank pass,
 //    which reproduces thrust's stable (cellId, attachId) order exactly;
@@ -390,7 +393,8 @@ __global__ void k_gather_u32(const uint32_t *__restrict__ src, const uint32_t *_
     if (i < n) dst[i] = src[idx[i]];
 }
 
-// Within one cell the radix sort leaves particles in previous-slot order;
+// Within one cell the sort leaves particles in previous-slot order (radix) or
+// in atomic order (counting);
 // rank them by attach id so the final order is (cellId, attachId), the order
 // thrust's stable sort of attach-ordered input produces.  Off-grid particles
 // share the dead key and keep the stable order.
