@@ -17,7 +17,10 @@
 //                      iteration/residual semantics on hand-built cases.
 //   IMM / IMM-CT     : sum of fractions = 1 after regularise, zero diffusion
 //                      for uniform fractions, zero drift flux for zero drift,
-//                      Q = I fixed point for L = 0, single-particle Q ODE.
+//                      Q = I fixed point for L = 0; ct_update by the simple-
+//                      shear steady state, the logistic solution of the
+//                      thinning term and the viscosity law's special points
+//                      (test_oracle_conformation.py).
 //   PBF adjoint      : central finite differences of this same FP64 forward.
 // Values the reference leaves undefined (uninitialised boundary lambda and
 // kappa_incomp, racy boundary rest density) are defined as zero / race-free.
@@ -78,6 +81,25 @@ inline V3 matvec(const M3 &a, V3 v) {  // Matrix.hpp:855-860
             a(2, 0) * v.x + a(2, 1) * v.y + a(2, 2) * v.z};
 }
 inline double trace(const M3 &a) { return a(0, 0) + a(1, 1) + a(2, 2); }
+
+// update_ct_cuda (IMMCTCudaImpl.cu:716-744) for one particle: explicit Euler
+// step of Q' = Q L + L^T Q - (Q - I)/relax - beta (Q - I) Q, the viscosity
+// mu = base + (min(base kappa, max) - base) / (1 + (10 gamma)^5) min(1, a1/(th + 1e-6))
+// with gamma = sqrt(tr(D)^2 / 2), D = (L + L^T)/2 (the reference's trace
+// squared), and the stress mu (Q - I).  Q and Qout may alias.
+inline void ct_update(const M3 &Q, const M3 &L, double dt, double relax, double beta, double kappa, double vis_base,
+                      double vis_max, double a1, double th, M3 &Qout, double &mu_out, M3 &stress) {
+    const M3 I = M3::eye();
+    const M3 dQ = (matmul(Q, L) + matmul(transpose(L), Q) - (1.0 / relax) * (Q - I) - matmul(beta * (Q - I), Q)) * dt;
+    const M3 Qn = Q + dQ;
+    const M3 D = 0.5 * (L + transpose(L));
+    const double shear = std::sqrt(0.5 * trace(D) * trace(D));
+    double mu = std::min(vis_base * kappa, vis_max);
+    mu = vis_base + (mu - vis_base) / (1 + std::pow(10 * shear, 5)) * std::min(1.0, a1 / (th + 1e-6));
+    Qout = Qn;
+    mu_out = mu;
+    stress = mu * (Qn - I);
+}
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr double kPi = 3.14159265;  // KernelFunctions.cuh:16 literal
@@ -740,6 +762,22 @@ extern "C" void orc_kernel(double h, const double *r, double *w, double *grad, d
 
 // The FP32 decision functions, exposed for the UGNS brute-force tests.
 extern "C" int orc_cell_coord(float x, float lb, float cell) { return cell_coord(x, lb, cell); }
+
+// One conformation update (row-major 3x3 arrays), for the closed-form pins.
+extern "C" void orc_ct_update(const double *Q, const double *L, double dt, double relax, double beta, double kappa,
+                              double vis_base, double vis_max, double a1, double th, double *Qout, double *mu,
+                              double *stress) {
+    M3 q, l, qo, st;
+    for (int k = 0; k < 9; ++k) {
+        q.m[k] = Q[k];
+        l.m[k] = L[k];
+    }
+    ct_update(q, l, dt, relax, beta, kappa, vis_base, vis_max, a1, th, qo, *mu, st);
+    for (int k = 0; k < 9; ++k) {
+        Qout[k] = qo.m[k];
+        stress[k] = st.m[k];
+    }
+}
 extern "C" int orc_neighbor_test(const float *a, const float *b, float cell) {
     return neighbor_test(V3{a[0], a[1], a[2]}, V3{b[0], b[1], b[2]}, cell) ? 1 : 0;
 }

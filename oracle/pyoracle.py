@@ -47,6 +47,8 @@ def load():
         lib.orc_kernel.restype = None
         lib.orc_cell_coord.argtypes = [C.c_float, C.c_float, C.c_float]
         lib.orc_neighbor_test.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float]
+        lib.orc_ct_update.argtypes = [_D, _D] + [C.c_double] * 8 + [_D, _D, _D]
+        lib.orc_ct_update.restype = None
         _lib = lib
     return _lib
 
@@ -175,6 +177,19 @@ def kernel(h: float, r):
     c = C.c_double()
     lib.orc_kernel(h, _dp(rr), C.byref(w), _dp(g), C.byref(c))
     return w.value, g, c.value
+
+
+def ct_update(Q, L, dt, relax, beta, kappa, vis_base, vis_max, a1, th):
+    """One conformation update of the oracle (update_ct_cuda): (Q_new, mu, stress)."""
+    lib = load()
+    q = np.ascontiguousarray(Q, np.float64).reshape(9)
+    l = np.ascontiguousarray(L, np.float64).reshape(9)
+    qo = np.zeros(9)
+    st = np.zeros(9)
+    mu = C.c_double()
+    lib.orc_ct_update(_dp(q), _dp(l), dt, relax, beta, kappa, vis_base, vis_max, a1, th, _dp(qo), C.byref(mu),
+                      _dp(st))
+    return qo.reshape(3, 3), mu.value, st.reshape(3, 3)
 
 
 def set_threads(n: int):
