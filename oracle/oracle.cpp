@@ -17,7 +17,9 @@
 //                      iteration/residual semantics on hand-built cases.
 //   IMM / IMM-CT     : sum of fractions = 1 after regularise, zero diffusion
 //                      for uniform fractions, zero drift flux for zero drift,
-//                      Q = I fixed point for L = 0; ct_update by the simple-
+//                      Q = I fixed point for L = 0; per-phase totals
+//                      conserved through ticks, diffusion down the gradient
+//                      (test_oracle_phase_transfer.py); ct_update by the simple-
 //                      shear steady state, the logistic solution of the
 //                      thinning term and the viscosity law's special points
 //                      (test_oracle_conformation.py).
