@@ -145,10 +145,10 @@ def test_resume_keeps_the_adaptive_list_walk():
     wl = scenes.Workload("grad", "PBF", scenes.pbf_config(lb=(0, 0, 0), size=(0.3, 0.3, 0.3), max_nbr=40),
                          [scenes.Obj(vtp.MAT_FLUID, pos)])
     a = _context(wl, "PBF")
-    a.step(3)
+    a.step(1)
     image = a.save_state()
     h, _ = vtp.parse_state(image)
-    assert h.nbr_at_cap > h.n // 32
+    assert h.nbr_at_cap > h.n // 8   # about half the lists: the next build takes the reference walk
     a.step(3)
     want = _snapshot(a, CARRIED["PBF"])
     b = _context(wl, "PBF")

@@ -42,7 +42,7 @@ constexpr uint32_t kNbrBuildThreads = 128;
 #define VTP_REF_BATCH 8
 #endif
 #ifndef VTP_REF_ORDER_SHARE
-#define VTP_REF_ORDER_SHARE 32
+#define VTP_REF_ORDER_SHARE 8
 #endif
 constexpr uint32_t kNbrStage = VTP_NBR_STAGE;   // list entries per thread staged in LDS (8 KB per workgroup)
 // A build walks every particle in the reference's cell order when more than
