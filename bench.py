@@ -292,6 +292,7 @@ def main():
     n, nf = wl.n, wl.n_fluid
     grid, cells = ctx.grid_info()
     ctx.step(args.warmup)
+    torch.cuda.synchronize()
     image = ctx.save_state()          # the state entering the timed ticks
     torch.cuda.synchronize()
     t0 = time.perf_counter()

@@ -88,6 +88,20 @@ std::vector<Section> sections(Ctx &c) {
     return s;
 }
 
+// Rows are copied one 1D hipMemcpyAsync each (a section has at most 9), which
+// behaves the same under every runtime interposer; a 2D copy into pageable
+// host memory came back as zeros under rocprofv3's kernel tracing.
+void copy_rows_d2h(unsigned char *host, const Section &s, hipStream_t st) {
+    for (size_t r = 0; r < s.rows; ++r)
+        VTP_HIP(hipMemcpyAsync(host + r * s.row_bytes, static_cast<const unsigned char *>(s.dev) + r * s.pitch_bytes,
+                               s.row_bytes, hipMemcpyDeviceToHost, st));
+}
+void copy_rows_h2d(const unsigned char *host, const Section &s, hipStream_t st) {
+    for (size_t r = 0; r < s.rows; ++r)
+        VTP_HIP(hipMemcpyAsync(static_cast<unsigned char *>(s.dev) + r * s.pitch_bytes, host + r * s.row_bytes,
+                               s.row_bytes, hipMemcpyHostToDevice, st));
+}
+
 size_t payload_bytes(Ctx &c) {
     size_t b = 0;
     for (const Section &s : sections(c)) b += s.bytes();
@@ -159,8 +173,7 @@ extern "C" vtp_status vtp_save_state(vtp_ctx *ctx, void *host_dst, size_t bytes)
         std::memcpy(out, &h, sizeof h);
         size_t off = sizeof h;
         for (const Section &s : sections(c)) {
-            VTP_HIP(hipMemcpy2DAsync(out + off, s.row_bytes, s.dev, s.pitch_bytes, s.row_bytes, s.rows,
-                                     hipMemcpyDeviceToHost, c.stream));
+            copy_rows_d2h(out + off, s, c.stream);
             off += s.bytes();
         }
         VTP_HIP(hipStreamSynchronize(c.stream));
@@ -202,8 +215,7 @@ extern "C" vtp_status vtp_load_state(vtp_ctx *ctx, const void *host_src, size_t 
         VTP_HIP(hipStreamSynchronize(c.stream));
         size_t off = sizeof h;
         for (const Section &s : sections(c)) {
-            VTP_HIP(hipMemcpy2DAsync(s.dev, s.pitch_bytes, in + off, s.row_bytes, s.row_bytes, s.rows,
-                                     hipMemcpyHostToDevice, c.stream));
+            copy_rows_h2d(in + off, s, c.stream);
             off += s.bytes();
         }
         hipLaunchKernelGGL(k_slot_tables, dim3(div_up(c.n, 256)), dim3(256), 0, c.stream, c.orig.p, c.mat_attach.p,
