@@ -73,13 +73,16 @@ __global__ __launch_bounds__(256, VTP_PASS_MIN_WAVES) void k_pbf_lambda(const fl
                                                     const uint32_t *__restrict__ cnt, uint32_t n,
                                                     uint32_t stride, PbfK q, float *__restrict__ lam,
                                                     float *__restrict__ dens, float *__restrict__ errc,
-                                                    float4 *__restrict__ sg, float4 *__restrict__ xl) {
+                                                    float4 *__restrict__ sg, float4 *__restrict__ xl,
+                                                    int static_out) {
     const uint32_t g = slot_index(), i = g / L, sub = g % L;   // L lanes per particle
     if (i >= n) return;
-    const float4 xi = x[i];
     if (mat[i] != VTP_MAT_FLUID) {
-        if (sub != 0) return;
-        // slots are permuted every tick, so non-fluid slots are written too
+        // slots are permuted every tick, so non-fluid slots are written too:
+        // by the first iteration of a tick (static particles do not move and
+        // the outputs are the same buffers), by every iteration of a taped one
+        if (sub != 0 || !static_out) return;
+        const float4 xi = x[i];
         lam[i] = 0.0f;
         dens[i] = 0.0f;
         errc[i] = 0.0f;
@@ -87,6 +90,7 @@ __global__ __launch_bounds__(256, VTP_PASS_MIN_WAVES) void k_pbf_lambda(const fl
         if (sg) sg[i] = make_float4(0, 0, 0, 0);
         return;
     }
+    const float4 xi = x[i];
     const uint32_t w = cnt[i];
     const uint32_t nn = w & ~kDeadBit;
     float rho = ((w & kDeadBit) || sub != 0) ? 0.0f : xi.w * q.k.w0;
@@ -319,7 +323,7 @@ static void pbf_enqueue_step(Ctx &c, PbfTapeStep *tape = nullptr) {
         const dim3 gl(div_up(n * L, bl)), gd(div_up(n * L, bd));
 #define VTP_LAMBDA(MW, LL)                                                                                     \
     hipLaunchKernelGGL((k_pbf_lambda<MW, LL>), gl, dim3(bl), 0, c.stream, cur, c.mat.p, u.nbr.p, u.cnt.p, n,    \
-                       u.stride, q, lam, c.dens.p, errc, sg, c.xl.p)
+                       u.stride, q, lam, c.dens.p, errc, sg, c.xl.p, (tape || it == 0) ? 1 : 0)
 #define VTP_DX(MW, LL)                                                                                         \
     hipLaunchKernelGGL((k_pbf_dx<MW, LL>), gd, dim3(bd), 0, c.stream, cur, c.xl.p, nxt, c.V[1].p, c.mat.p,    \
                        u.nbr.p, u.cnt.p, n, u.stride, q, tape ? 1 : 0)
