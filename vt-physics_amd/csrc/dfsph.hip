@@ -196,7 +196,8 @@ __global__ __launch_bounds__(256) void k_df_delta_kappa(const float4 *__restrict
                                                         const uint32_t *__restrict__ cnt, uint32_t n,
                                                         uint32_t stride, Cubic k, const float *__restrict__ cr,
                                                         const float *__restrict__ alpha, float *__restrict__ delta,
-                                                        float *__restrict__ kappa, float4 *__restrict__ xk, Loop lp) {
+                                                        float *__restrict__ kappa, float4 *__restrict__ xk, Loop lp,
+                                                        int static_out) {
     if (lp.status[kStDone]) return;
     const uint32_t i = slot_index();
     const float dt = lp.sc->dt;
@@ -225,7 +226,9 @@ __global__ __launch_bounds__(256) void k_df_delta_kappa(const float4 *__restrict
             kappa[i] = ki;
             xk[i] = make_float4(xi.x, xi.y, xi.z, Vi * ki);
             contrib = (double)dl * lp.obj_w[obj_of[i]];
-        } else {
+        } else if (static_out) {
+            // static particles do not move during a solve: their kappa and
+            // (x, 0) records are written by its first iteration only
             kappa[i] = 0.0f;
             const float4 xi = x[i];
             xk[i] = make_float4(xi.x, xi.y, xi.z, 0.0f);
@@ -490,7 +493,7 @@ void df_pressure_solve(Ctx &c, const float4 *x, const float4 *v, int which) {
             c.timer.begin(c.stream, name_dk);
             hipLaunchKernelGGL(k_df_delta_kappa, dim3(div_up(n, bd)), dim3(bd), 0, c.stream, x, v, c.vadv.p, c.mat.p,
                                c.obj_of.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, c.cr.p, c.alpha.p, c.delta.p,
-                               c.kappa.p, c.xk.p, lp);
+                               c.kappa.p, c.xk.p, lp, launched == 0 ? 1 : 0);
             c.timer.end(c.stream);
             c.timer.begin(c.stream, name_up);
             hipLaunchKernelGGL(k_df_update_vadv, dim3(div_up(n, bu)), dim3(bu), 0, c.stream, x, v, c.xk.p, c.g1.p,
