@@ -40,8 +40,8 @@ def kernel_bytes(name, n, nf):
         return 24 * n + 48 * nf
     if name == "pbf_xsph":        # x, v, mat in (all); x, v out (all)
         return 68 * n
-    if name == "ugns_neighbors":  # x + key in; count word out
-        return 28 * n
+    if name == "ugns_neighbors":  # x + key in; count word out; with the tick's first lambda pass
+        return 36 * n + 8 * nf    # mat in, lambda out (all), rho and C out (fluid)
     return None
 
 

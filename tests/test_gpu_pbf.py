@@ -134,6 +134,57 @@ def test_pbf_counting_and_radix_sort_bit_identical():
     assert max(dead) > 0 and len(set(dead)) > 1
 
 
+@pytest.mark.parametrize("mass_weighted,walk", [(False, 0), (True, 0), (False, 2)])
+def test_pbf_lambda_from_the_build_bit_identical(mass_weighted, walk):
+    """By default the neighbour build computes the first iteration's density
+    and lambda pass while it writes each list, with the pass's per-neighbour
+    code in list order.  Ticks are bit-identical to running the pass: with
+    particles that leave the grid (dead), lists cut by the cap (rebuilt in
+    the reference's order), the reference walk, the mass-weighted gradient, a
+    tick that also builds the static particles' lists, and a taped rollout
+    (mass-free gradient: the one the adjoint differentiates)."""
+    wl = scenes.cfg0_pbf(n_side=(16, 16, 16))
+    wl.config["PBF"].setdefault("Extensions", {})["massWeighted"] = mass_weighted
+    rng = np.random.default_rng(29)
+    jet = rng.uniform(0.3, 0.7, size=(400, 3)).astype(np.float32)
+    jet[:, 1] = rng.uniform(0.9, 1.05, 400)
+    vel = np.zeros_like(jet)
+    vel[:, 1] = rng.uniform(1.0, 3.0, 400)
+    wl.objects.append(scenes.Obj(vtp.MAT_FLUID, jet, vel))
+    # a crushed clump: lists past the cap in the first tick
+    clump = (rng.uniform(-0.02, 0.02, size=(150, 3)) + [0.5, 0.05, 0.5]).astype(np.float32)
+    wl.objects.append(scenes.Obj(vtp.MAT_FLUID, clump, None))
+    a, _ = _pair(wl, graph=1, neighbor_walk=walk)
+    b, _ = _pair(wl, graph=1, neighbor_walk=walk, build_lambda=0)
+    fields = ("pos", "vel", "density", "lambda")
+    dead = 0
+    for r in range(4):
+        static = 1 if r == 0 else 0   # the first tick also builds the walls' lists (exportable)
+        a.set_option("static_lists", static)
+        b.set_option("static_lists", static)
+        a.step(1 if r == 0 else 6)
+        b.step(1 if r == 0 else 6)
+        for f in fields:
+            np.testing.assert_array_equal(a.read(f), b.read(f), err_msg=f)
+        if r == 0:   # the clump's lists were cut by the cap
+            assert a.debug_neighbors()["neighbor_num"].max() >= a.params.max_neighbor_num
+        dead = max(dead, a.stats()["dead_particles"])
+    assert dead > 0
+    if mass_weighted:
+        return   # the adjoint differentiates the reference (mass-free) gradient only
+    nf = wl.n_fluid
+    cx = np.zeros((wl.n, 3))
+    cx[:nf] = np.random.default_rng(7).normal(size=(nf, 3))
+    grads = []
+    for ctx in (a, b):
+        ctx.rollout_fwd(4, 2)
+        grads.append(ctx.rollout_bwd(cx, None))
+    np.testing.assert_array_equal(a.read("pos"), b.read("pos"))
+    for ga, gb in zip(grads[0][:2], grads[1][:2]):
+        np.testing.assert_allclose(ga, gb, rtol=1e-12, atol=0)
+    assert abs(grads[0][2] - grads[1][2]) <= 1e-12 * abs(grads[1][2])
+
+
 def test_pbf_autotune_keeps_results_bit_identical():
     """The autotuner only changes workgroup sizes (per-particle results do not
     depend on them) and restores the state it timed from."""

@@ -635,6 +635,9 @@ extern "C" vtp_status vtp_set_option(vtp_ctx *ctx, const char *key, double value
             pbf_capture_reset(c);
         } else if (k == "full_flux_retry") {
             c.opt_full_flux_retry = value != 0.0;
+        } else if (k == "build_lambda") {
+            c.opt_build_lambda = value != 0.0;
+            pbf_capture_reset(c);
         } else if (k == "radix_sort") {
             c.ugns.radix = value != 0.0;
             pbf_capture_reset(c);

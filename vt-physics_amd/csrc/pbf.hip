@@ -9,7 +9,10 @@
 //   * delta-x and apply_dx are one pass writing into a second position
 //     buffer (Jacobi semantics preserved: every delta-x reads the positions
 //     of the previous iteration), XSPH and its apply likewise;
-//   * the predict (apply_ext_force) is fused with the cell hash.
+//   * the predict (apply_ext_force) is fused with the cell hash;
+//   * the first iteration's density+lambda pass is computed by the neighbour
+//     build (one lane per particle), which holds every listed neighbour's
+//     position and mass as it writes the list.
 // Boundary particles never move: their positions sit in both ping-pong
 // buffers, so the passes skip them entirely.
 #include "vtp_internal.h"
@@ -79,15 +82,11 @@ __global__ __launch_bounds__(256, VTP_PASS_MIN_WAVES) void k_pbf_lambda(const fl
     if (i >= n) return;
     if (mat[i] != VTP_MAT_FLUID) {
         // slots are permuted every tick, so non-fluid slots are written too:
-        // by the first iteration of a tick (static particles do not move and
-        // the outputs are the same buffers), by every iteration of a taped one
+        // by the first iteration of a tick, or the build that computes it
+        // (static particles do not move and the outputs are the same
+        // buffers), and by every iteration of a taped one
         if (sub != 0 || !static_out) return;
-        const float4 xi = x[i];
-        lam[i] = 0.0f;
-        dens[i] = 0.0f;
-        errc[i] = 0.0f;
-        xl[i] = make_float4(xi.x, xi.y, xi.z, 0.0f);
-        if (sg) sg[i] = make_float4(0, 0, 0, 0);
+        pbf_lambda_store_static(i, x[i], lam, dens, errc, sg, xl);
         return;
     }
     const float4 xi = x[i];
@@ -97,37 +96,14 @@ __global__ __launch_bounds__(256, VTP_PASS_MIN_WAVES) void k_pbf_lambda(const fl
     float gsx = 0, gsy = 0, gsz = 0, s2 = 0;
     for_each_neighbor_coop<L>(
         nbr, stride, i, nn, sub, [&](uint32_t j) { return ldg(x, j); },
-        [&](const float4 &xj) {
-            float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            float W, f;
-            cubic_w_grad(r2, q.k, W, f);
-            rho = fmaf(xj.w, W, rho);
-            if (MW) f *= xj.w;
-            gsx = fmaf(dx, f, gsx);
-            gsy = fmaf(dy, f, gsy);
-            gsz = fmaf(dz, f, gsz);
-            s2 = fmaf(f * f, r2, s2);   // |f r_vec|^2
-        });
+        [&](const float4 &xj) { pbf_lambda_pair<MW>(xi, xj, q.k, rho, gsx, gsy, gsz, s2); });
     rho = lane_sum<L>(rho);
     gsx = lane_sum<L>(gsx);
     gsy = lane_sum<L>(gsy);
     gsz = lane_sum<L>(gsz);
     s2 = lane_sum<L>(s2);
     if (sub != 0) return;
-    rho = fmaxf(rho, q.rho0);
-    // rho/rho0 - 1 with a rounded division, as the reference (PBFCudaImpl.cu:92):
-    // a clamped particle gets exactly C = 0 (a fused rho*(1/rho0) - 1 would
-    // leave 1000*0.001f - 1 = 4.7e-8, and the adjoint tests C > 0 for the clamp).
-    float C = __fdiv_rn(rho, q.rho0) - 1.0f;
-    float ir2 = q.inv_rho0 * q.inv_rho0;
-    float S = (s2 + gsx * gsx + gsy * gsy + gsz * gsz) * ir2;
-    const float li = -C / (S + 1e-6f);
-    lam[i] = li;
-    xl[i] = make_float4(xi.x, xi.y, xi.z, li);   // the delta-x pass gathers (x_j, lambda_j) in one load
-    dens[i] = rho;
-    errc[i] = C;
-    if (sg) sg[i] = make_float4(gsx, gsy, gsz, S);   // taped for the adjoint
+    pbf_lambda_store(i, xi, rho, gsx, gsy, gsz, s2, q, lam, dens, errc, sg, xl);
 }
 
 // compute_dx_cuda + apply_dx_cuda (PBFCudaImpl.cu:124-170):
@@ -308,14 +284,19 @@ static void pbf_enqueue_step(Ctx &c, PbfTapeStep *tape = nullptr) {
     float4 *const x0 = tape ? tape->x[0] : c.X[1].p;
     common_reorder(c, c.X[0].p, x0, tape ? nullptr : c.X[2].p, c.V[0].p, c.V[1].p, nullptr);
     c.timer.end(c.stream);
+    const uint32_t L = c.opt_lanes;
+    // The build computes the first iteration's density + lambda pass while it
+    // holds each listed neighbour (one lane per particle: the same sums in
+    // the same order as the pass).
+    const bool build_lambda = c.opt_build_lambda && L == 1 && c.p.iteration_num > 0;
+    const PbfLambdaOut lo{q, c.mat.p, tape ? tape->lam[0] : c.lam.p, c.dens.p, tape ? tape->C[0] : c.err_c.p,
+                          tape ? tape->sg[0] : nullptr, c.xl.p};
     // no kernel of a PBF tick (nor its adjoint) walks a static particle's list
-    u.build_neighbors(x0, c.stream, c.timer, c.opt_static_lists ? nullptr : c.mat.p);
+    u.build_neighbors(x0, c.stream, c.timer, c.opt_static_lists ? nullptr : c.mat.p, build_lambda ? &lo : nullptr);
     if (tape) VTP_HIP(hipMemcpyAsync(tape->orig, c.orig.p, n * 4, hipMemcpyDeviceToDevice, c.stream));
     float4 *cur = x0, *nxt = c.X[2].p;
     const uint32_t bl = c.block("pbf_lambda"), bd = c.block("pbf_dx"), bx = c.block("pbf_xsph");
-    const uint32_t L = c.opt_lanes;
     for (uint32_t it = 0; it < c.p.iteration_num; ++it) {
-        c.timer.begin(c.stream, "pbf_lambda");
         float *lam = tape ? tape->lam[it] : c.lam.p;
         float *errc = tape ? tape->C[it] : c.err_c.p;
         float4 *sg = tape ? tape->sg[it] : nullptr;
@@ -327,9 +308,12 @@ static void pbf_enqueue_step(Ctx &c, PbfTapeStep *tape = nullptr) {
 #define VTP_DX(MW, LL)                                                                                         \
     hipLaunchKernelGGL((k_pbf_dx<MW, LL>), gd, dim3(bd), 0, c.stream, cur, c.xl.p, nxt, c.V[1].p, c.mat.p,    \
                        u.nbr.p, u.cnt.p, n, u.stride, q, tape ? 1 : 0)
-        if (q.mass_weighted) { if (L == 4) VTP_LAMBDA(true, 4); else VTP_LAMBDA(true, 1); }
-        else { if (L == 4) VTP_LAMBDA(false, 4); else VTP_LAMBDA(false, 1); }
-        c.timer.end(c.stream);
+        if (it > 0 || !build_lambda) {
+            c.timer.begin(c.stream, "pbf_lambda");
+            if (q.mass_weighted) { if (L == 4) VTP_LAMBDA(true, 4); else VTP_LAMBDA(true, 1); }
+            else { if (L == 4) VTP_LAMBDA(false, 4); else VTP_LAMBDA(false, 1); }
+            c.timer.end(c.stream);
+        }
         c.timer.begin(c.stream, "pbf_dx");
         if (q.mass_weighted) { if (L == 4) VTP_DX(true, 4); else VTP_DX(true, 1); }
         else { if (L == 4) VTP_DX(false, 4); else VTP_DX(false, 1); }

@@ -38,6 +38,15 @@ constexpr uint32_t kNbrBuildThreads = 128;
 #ifndef VTP_BUILD_BATCH
 #define VTP_BUILD_BATCH 8
 #endif
+#ifndef VTP_NBR_STAGE_LAMBDA
+#define VTP_NBR_STAGE_LAMBDA 24
+#endif
+#ifndef VTP_BUILD_MIN_WAVES
+#define VTP_BUILD_MIN_WAVES 6
+#endif
+#ifndef VTP_BUILD_BATCH_LAMBDA
+#define VTP_BUILD_BATCH_LAMBDA 8
+#endif
 #ifndef VTP_REF_BATCH
 #define VTP_REF_BATCH 8
 #endif
@@ -45,6 +54,9 @@ constexpr uint32_t kNbrBuildThreads = 128;
 #define VTP_REF_ORDER_SHARE 8
 #endif
 constexpr uint32_t kNbrStage = VTP_NBR_STAGE;   // list entries per thread staged in LDS (8 KB per workgroup)
+// The PBF build that also computes lambda needs more registers (6 waves per
+// SIMD), which leaves room for a larger stage at the same occupancy.
+constexpr uint32_t kNbrStageLambda = VTP_NBR_STAGE_LAMBDA;   // 12 KB per workgroup
 // A build walks every particle in the reference's cell order when more than
 // 1/kRefOrderShare of the particles reached the cap in the previous build.
 constexpr uint32_t kRefOrderShare = VTP_REF_ORDER_SHARE;
@@ -433,7 +445,7 @@ __device__ uint32_t reference_scan(const float4 *__restrict__ x, const uint32_t 
     // j == i gives d2 == 0, which the test rejects (the reference skips it)
     auto test = [&](uint32_t j, const float4 &xj) {
         if (is_neighbor(dist2_exact(p, xyz(xj)), g.nbr_d2_max)) {
-            if (c < cap) put(c, j);
+            if (c < cap) put(c, j, xj);
             ++c;
         }
     };
@@ -489,8 +501,13 @@ __device__ uint32_t reference_scan(const float4 *__restrict__ x, const uint32_t 
 // differs, which changes FP32 rounding in the neighbour sums, so the count is
 // part of the carried state.
 //
-template <bool kStaged>
-__global__ __launch_bounds__(kNbrBuildThreads) void k_build_neighbors(const float4 *__restrict__ x,
+// kLam = 1 (2: mass-weighted) also accumulates the density and lambda sums
+// of a PBF tick's first constraint iteration over each list as it is
+// written (the walk holds every listed neighbour's position and mass), with
+// the lambda pass's own per-neighbour code and order, and stores the pass's
+// outputs, static particles' included.
+template <bool kStaged, int kLam>
+__global__ __launch_bounds__(kNbrBuildThreads, VTP_BUILD_MIN_WAVES) void k_build_neighbors(const float4 *__restrict__ x,
                                                                       const uint32_t *__restrict__ key,
                                                                       const uint32_t *__restrict__ start,
                                                                       const uint32_t *__restrict__ end, Grid g,
@@ -499,22 +516,38 @@ __global__ __launch_bounds__(kNbrBuildThreads) void k_build_neighbors(const floa
                                                                       uint32_t *__restrict__ cnt,
                                                                       uint32_t *__restrict__ status,
                                                                       uint32_t walk,
-                                                                      const int32_t *__restrict__ fluid_only) {
+                                                                      const int32_t *__restrict__ fluid_only,
+                                                                      PbfLambdaOut lo) {
     extern __shared__ uint32_t s_list[];
     const uint32_t i = slot_index();
     // uniform over the grid; walk = 1 / 2 forces the row / reference walk
     const bool ref_walk = walk == 2 || (walk == 0 && status[kStNbrPrev] > n / kRefOrderShare);
-    const uint32_t stage = kStaged ? min(cap, kNbrStage) : 0u;
-    auto put = [&](uint32_t c, uint32_t j) {
+    const uint32_t stage = kStaged ? min(cap, kLam ? kNbrStageLambda : kNbrStage) : 0u;
+    float4 xi{};
+    float rho = 0, gsx = 0, gsy = 0, gsz = 0, s2 = 0;
+    bool sums = false;   // this particle's lambda sums are accumulated
+    auto put = [&](uint32_t c, uint32_t j, const float4 &xj) {
         if (c < stage) s_list[c * blockDim.x + threadIdx.x] = j;
         else nbr[(size_t)c * stride + i] = j;
+        if (kLam && sums) pbf_lambda_pair<kLam == 2>(xi, xj, lo.q.k, rho, gsx, gsy, gsz, s2);
+    };
+    auto restart = [&]() {   // the self term first, as the lambda pass
+        rho = xi.w * lo.q.k.w0;
+        gsx = gsy = gsz = s2 = 0.0f;
     };
     uint32_t c = 0;
     bool full = false;
     if (i < n) {
         const uint32_t k = key[i];
+        bool fluid = false;
+        if (kLam) {
+            xi = x[i];
+            fluid = lo.mat[i] == VTP_MAT_FLUID;
+        }
         if (k >= g.dead_key) {
             cnt[i] = kDeadBit;
+            if (kLam && fluid)   // no neighbours and no self term (count word's dead bit)
+                pbf_lambda_store(i, xi, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, lo.q, lo.lam, lo.dens, lo.errc, lo.sg, lo.xl);
         } else if (fluid_only && fluid_only[i] != VTP_MAT_FLUID) {
             cnt[i] = 0;   // a static particle whose list no kernel of this tick reads
         } else {
@@ -526,16 +559,18 @@ __global__ __launch_bounds__(kNbrBuildThreads) void k_build_neighbors(const floa
             const int cy = (int)(rem / (uint32_t)g.gx);
             const int cx = (int)(rem - (uint32_t)cy * (uint32_t)g.gx);
             const uint32_t sbase = scene * g.cells;
-            const float3 p = xyz(x[i]);
+            const float3 p = kLam ? xyz(xi) : xyz(x[i]);
+            sums = kLam && fluid;
+            if (sums) restart();
             auto test = [&](uint32_t j, const float4 &xj) {
                 // j == i gives d2 == 0, which the test rejects
                 if (is_neighbor(dist2_exact(p, xyz(xj)), g.nbr_d2_max)) {
-                    if (c < cap) put(c, j);
+                    if (c < cap) put(c, j, xj);
                     ++c;
                 }
             };
             auto walk = [&](uint32_t j, uint32_t e) {
-                constexpr uint32_t B = VTP_BUILD_BATCH;   // candidates in flight
+                constexpr uint32_t B = kLam ? VTP_BUILD_BATCH_LAMBDA : VTP_BUILD_BATCH;   // candidates in flight
                 for (; j <= e; j += B) {
                     float4 a[B];
 #pragma unroll
@@ -581,11 +616,16 @@ __global__ __launch_bounds__(kNbrBuildThreads) void k_build_neighbors(const floa
                     if (r < 8) request(r + 1);
                     if (s != kEmpty && c <= cap) walk(s, e);   // past the cap: rebuilt below
                 }
-                if (c > cap) c = reference_scan(x, start, end, g, p, cx, cy, cz, sbase, cap, put);
+                if (c > cap) {
+                    if (sums) restart();
+                    c = reference_scan(x, start, end, g, p, cx, cy, cz, sbase, cap, put);
+                }
             }
             cnt[i] = c;
             full = cap > 0 && c == cap;
+            if (sums) pbf_lambda_store(i, xi, rho, gsx, gsy, gsz, s2, lo.q, lo.lam, lo.dens, lo.errc, lo.sg, lo.xl);
         }
+        if (kLam && !fluid) pbf_lambda_store_static(i, xi, lo.lam, lo.dens, lo.errc, lo.sg, lo.xl);
     }
     // particles that reached the cap steer the walk of the next build
     const uint64_t m = __ballot(full);
@@ -735,13 +775,20 @@ void Ugns::sort_and_rank(const uint32_t *orig_old, hipStream_t s, StageTimer &tm
     tm.end(s);
 }
 
-void Ugns::build_neighbors(const float4 *x, hipStream_t s, StageTimer &tm, const int32_t *fluid_only) {
+void Ugns::build_neighbors(const float4 *x, hipStream_t s, StageTimer &tm, const int32_t *fluid_only,
+                           const PbfLambdaOut *lam) {
     if (!status) throw std::runtime_error("neighbour build without status words");
     static_rows = fluid_only == nullptr;
     tm.begin(s, "ugns_neighbors");
-    const size_t lds = (size_t)std::min(cap, kNbrStage) * kNbrBuildThreads * sizeof(uint32_t);
-    hipLaunchKernelGGL(k_build_neighbors<true>, dim3(blocks_for(n, kNbrBuildThreads)), dim3(kNbrBuildThreads), lds,
-                       s, x, key[sorted].p, start.p, end.p, g, n, stride, cap, nbr.p, cnt.p, status, walk, fluid_only);
+    const size_t lds = (size_t)std::min(cap, lam ? kNbrStageLambda : kNbrStage) * kNbrBuildThreads * sizeof(uint32_t);
+    const dim3 grid(blocks_for(n, kNbrBuildThreads)), block(kNbrBuildThreads);
+#define VTP_BUILD(LAM, OUT)                                                                                   \
+    hipLaunchKernelGGL((k_build_neighbors<true, LAM>), grid, block, lds, s, x, key[sorted].p, start.p, end.p, g, \
+                       n, stride, cap, nbr.p, cnt.p, status, walk, fluid_only, OUT)
+    if (!lam) VTP_BUILD(0, PbfLambdaOut{});
+    else if (lam->q.mass_weighted) VTP_BUILD(2, *lam);
+    else VTP_BUILD(1, *lam);
+#undef VTP_BUILD
     tm.end(s);
 }
 

@@ -106,6 +106,8 @@ struct StageTimer {
 };
 
 // Uniform-grid neighbour search state (UniformGridNeighborSearch.hpp:28-63).
+struct PbfLambdaOut;
+
 struct Ugns {
     Grid g{};
     uint32_t n = 0, M = 0, cap = 0, stride = 0;
@@ -124,8 +126,11 @@ struct Ugns {
     void allocate(uint32_t n, const Grid &g, uint32_t max_neighbors);
     void sort_and_rank(const uint32_t *orig_old, hipStream_t s, StageTimer &tm);
     // fluid_only = per-slot materials: the lists of non-fluid particles are
-    // left empty (count 0); nullptr builds every list, as the reference does
-    void build_neighbors(const float4 *x, hipStream_t s, StageTimer &tm, const int32_t *fluid_only = nullptr);
+    // left empty (count 0); nullptr builds every list, as the reference does.
+    // lam: also compute the first density + lambda pass of a PBF tick from
+    // the neighbours as they are listed, and store it there.
+    void build_neighbors(const float4 *x, hipStream_t s, StageTimer &tm, const int32_t *fluid_only = nullptr,
+                         const PbfLambdaOut *lam = nullptr);
     void export_reference(const uint32_t *orig, uint32_t *d_lists, uint32_t *d_num, hipStream_t s);
     const uint32_t *sorted_key() const { return key[sorted].p; }
 };
@@ -172,6 +177,65 @@ struct PbfK {
     float gx, gy, gz;
     int mass_weighted;   // extension: m_j in the constraint gradient
 };
+
+// Where the first density + lambda pass of a PBF tick stores its results
+// when the neighbour build computes it (Ugns::build_neighbors).
+struct PbfLambdaOut {
+    PbfK q;
+    const int32_t *mat;
+    float *lam, *dens, *errc;
+    float4 *sg, *xl;   // sg may be null (untaped tick)
+};
+
+// One neighbour's terms of the density and lambda sums (PBFCudaImpl.cu:70-122).
+// The lambda pass and the build call it per neighbour in list order, so both
+// produce the same bits.  The multiply-adds are spelled out as fmaf: in the
+// build the products dx*dx.. also feed the uncontracted cut-off test, and
+// left to the compiler the two contexts could contract r2 differently.
+template <bool MW>
+__device__ __forceinline__ void pbf_lambda_pair(const float4 &xi, const float4 &xj, const Cubic &k, float &rho,
+                                                float &gsx, float &gsy, float &gsz, float &s2) {
+    float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    float W, f;
+    cubic_w_grad(r2, k, W, f);
+    rho = fmaf(xj.w, W, rho);
+    if (MW) f *= xj.w;
+    gsx = fmaf(dx, f, gsx);
+    gsy = fmaf(dy, f, gsy);
+    gsz = fmaf(dz, f, gsz);
+    s2 = fmaf(f * f, r2, s2);   // |f r_vec|^2
+}
+
+//   rho_i = max(rho, rho0),  C_i = rho_i/rho0 - 1,
+//   S_i = (sum_j |gradW_ij|^2 + |sum_j gradW_ij|^2) / rho0^2,  lambda_i = -C_i / (S_i + 1e-6).
+__device__ __forceinline__ void pbf_lambda_store(uint32_t i, const float4 &xi, float rho, float gsx, float gsy,
+                                                 float gsz, float s2, const PbfK &q, float *lam, float *dens,
+                                                 float *errc, float4 *sg, float4 *xl) {
+    rho = fmaxf(rho, q.rho0);
+    // rho/rho0 - 1 with a rounded division, as the reference (PBFCudaImpl.cu:92):
+    // a clamped particle gets exactly C = 0 (a fused rho*(1/rho0) - 1 would
+    // leave 1000*0.001f - 1 = 4.7e-8, and the adjoint tests C > 0 for the clamp).
+    float C = __fdiv_rn(rho, q.rho0) - 1.0f;
+    float ir2 = q.inv_rho0 * q.inv_rho0;
+    float S = fmaf(gsz, gsz, fmaf(gsy, gsy, fmaf(gsx, gsx, s2))) * ir2;
+    const float li = -C / (S + 1e-6f);
+    lam[i] = li;
+    xl[i] = make_float4(xi.x, xi.y, xi.z, li);   // the delta-x pass gathers (x_j, lambda_j) in one load
+    dens[i] = rho;
+    errc[i] = C;
+    if (sg) sg[i] = make_float4(gsx, gsy, gsz, S);   // taped for the adjoint
+}
+
+// A static particle's outputs: lambda = 0 (the reference never writes it).
+__device__ __forceinline__ void pbf_lambda_store_static(uint32_t i, const float4 &xi, float *lam, float *dens,
+                                                        float *errc, float4 *sg, float4 *xl) {
+    lam[i] = 0.0f;
+    dens[i] = 0.0f;
+    errc[i] = 0.0f;
+    xl[i] = make_float4(xi.x, xi.y, xi.z, 0.0f);
+    if (sg) sg[i] = make_float4(0, 0, 0, 0);
+}
 
 // Device buffers one taped PBF tick writes (owned by the adjoint state).
 struct PbfTapeStep {
@@ -232,6 +296,7 @@ struct Ctx {
     uint32_t opt_lanes = 1;   // lanes per particle in the PBF neighbour passes (1 or 4)
     bool opt_static_lists = false;   // PBF ticks also build the (unread) lists of static particles
     bool opt_full_flux_retry = false;  // IMM: every negative-fraction retry recomputes all fluxes
+    bool opt_build_lambda = true;   // PBF: the list build also computes the first lambda pass of a tick
     StageTimer timer;
 
     // attach-order tables (static)
