@@ -319,7 +319,10 @@ vtp_status vtp_load_state(vtp_ctx *ctx, const void *host_src, size_t bytes);
  * the same (cellId, attach id) order and bit-identical results),
  * "full_flux_retry" (IMM / IMM-CT: 0 = default, a negative-fraction retry
  * recomputes the fluxes of the particles next to newly flagged ones; 1 =
- * every retry recomputes all fluxes, as the reference.  Bit-identical). */
+ * every retry recomputes all fluxes, as the reference.  Bit-identical),
+ * "build_lambda" (PBF with "lanes" 1: 1 = default, the neighbour build also
+ * computes the first constraint iteration's density and lambda from the
+ * neighbours it lists; 0 = a separate pass does.  Bit-identical). */
 vtp_status vtp_set_option(vtp_ctx *ctx, const char *key, double value);
 
 /* Per-stage device time (ms, summed over launches) and launch counts,
