@@ -41,6 +41,9 @@ constexpr uint32_t kNbrBuildThreads = 128;
 #ifndef VTP_NBR_STAGE_LAMBDA
 #define VTP_NBR_STAGE_LAMBDA 24
 #endif
+// At least 6 waves per SIMD: caps the build at 80 VGPRs.  The plain build
+// needs 59 (8 waves); the one that also sums lambda would take 81 (5 waves)
+// and instead spills one register.
 #ifndef VTP_BUILD_MIN_WAVES
 #define VTP_BUILD_MIN_WAVES 6
 #endif
