@@ -328,6 +328,20 @@ def main():
                 roof["traffic"] = t["bytes"]
                 roof["traffic_GBps"] = round(t["bytes"] / (dom_ms * 1e-3) / 1e9, 1)
                 roof["traffic_source"] = str(TRAFFIC_JSON.relative_to(ROOT))
+    # the same figures for every neighbour kernel of the tick (us per launch,
+    # algorithmic and counter-measured fraction of the HBM peak)
+    traffic = json.loads(TRAFFIC_JSON.read_text())["kernels"] if TRAFFIC_JSON.exists() else {}
+    roof_all = {}
+    for name in STAGE_KERNEL:
+        if name not in stages or not stages[name][1]:
+            continue
+        ms = stages[name][0] / stages[name][1]
+        ab = kernel_bytes(name, n, nf)
+        e = {"us_per_launch": round(ms * 1e3, 2), "launches_per_tick": round(stages[name][1] / args.steps, 2),
+             "frac": round(ab / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
+        if name in traffic:
+            e["traffic_frac"] = round(traffic[name]["bytes"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)
+        roof_all[name] = e
     passes = 3 if cells * wl.scenes() >= (1 << 16) else 2
     step_bytes = pbf_step_bytes(n, nf, cells * wl.scenes(), params.iteration_num, passes)
     st = ctx.stats()
@@ -365,6 +379,7 @@ def main():
                    "grid": list(grid), "iterations": params.iteration_num, "dt": params.time_step,
                    "graph": True},
         "roofline": roof,
+        "roofline_by_kernel": roof_all,
         "cpu_baseline": cpu,
         "step_hbm_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
         "stages_ms_per_step": {k: round(v[0] / args.steps, 5) for k, v in stages.items()},
