@@ -44,7 +44,9 @@ typedef enum vtp_status {
     VTP_EHIP = 3,     /* HIP runtime error (launch, copy, ...)              */
     VTP_ECRASH = 4,   /* solver crashed: iteration cap, NaN, out-of-grid    */
     VTP_ESTATE = 5,   /* call out of order (e.g. step before initialize)    */
-    VTP_ENODEV = 6    /* no usable GPU / library built without device code  */
+    VTP_ENODEV = 6,   /* no usable GPU / library built without device code  */
+    VTP_ENBRCAP = 7   /* a neighbour list reached maxNeighborNum where the call
+                         needs complete (symmetric) lists: the PBF adjoint    */
 } vtp_status;
 
 /* Solver.hpp:20-25 (eSolverType). */

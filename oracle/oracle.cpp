@@ -720,6 +720,15 @@ extern "C" int orc_write(orc_sim *o, int32_t field, const double *src) {
     return 0;
 }
 
+// A neighbour search of the current positions alone (UGNSCudaImpl.cu:205-227),
+// without a solver step: the per-tick search of a device run can be compared
+// after its positions are written here.
+extern "C" int orc_search(orc_sim *o) {
+    if (!o) return -1;
+    ugns_update(o->s);
+    return 0;
+}
+
 extern "C" int orc_neighbors(orc_sim *o, uint32_t *lists, uint32_t *num, uint32_t *sorted, uint32_t *cell_start,
                              uint32_t *cell_end) {
     if (!o) return -1;

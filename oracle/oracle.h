@@ -41,6 +41,8 @@ int orc_write(orc_sim *s, int32_t field, const double *src);
 /* Reference-layout UGNS state of the last search (sizes N*M, N, N, C, C). */
 int orc_neighbors(orc_sim *s, uint32_t *lists, uint32_t *num, uint32_t *sorted, uint32_t *cell_start,
                   uint32_t *cell_end);
+/* Neighbour search of the current positions only (no step). */
+int orc_search(orc_sim *s);
 /* iterations of the last step: div, incomp, phase; residuals at exit. */
 int orc_stats(orc_sim *s, uint32_t *div_it, uint32_t *inc_it, uint32_t *phase_it, double *div_res,
               double *inc_res, uint32_t *crash);

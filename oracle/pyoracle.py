@@ -33,6 +33,7 @@ def load():
         lib.orc_read.argtypes = [C.c_void_p, C.c_int32, _D]
         lib.orc_write.argtypes = [C.c_void_p, C.c_int32, _D]
         lib.orc_neighbors.argtypes = [C.c_void_p, _U, _U, _U, _U, _U]
+        lib.orc_search.argtypes = [C.c_void_p]
         lib.orc_stats.argtypes = [C.c_void_p, _U, _U, _U, _D, _D, _U]
         lib.orc_destroy.argtypes = [C.c_void_p]
         lib.orc_destroy.restype = None
@@ -132,6 +133,11 @@ class Oracle:
         self.lib.orc_neighbors(self.h, _up(lists), _up(num), _up(srt), _up(cs), _up(ce))
         return {"neighbors": lists.reshape(self.n, M), "neighbor_num": num, "sorted": srt,
                 "cell_start": cs, "cell_end": ce}
+
+    def search(self):
+        """Neighbour search of the current positions alone (no step)."""
+        if self.lib.orc_search(self.h) != 0:
+            raise RuntimeError("orc_search failed")
 
     def stats(self):
         a, b, c, e = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()

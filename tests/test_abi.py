@@ -125,6 +125,23 @@ def test_cfl_extension():
     assert p.cfl_enable == 1 and np.isclose(p.cfl_number, 0.4) and np.isclose(p.dt_max, 0.002)
 
 
+@pytest.mark.parametrize("solver", ["PBF", "IMM", "IMMCT"])
+def test_cfl_extension_is_rejected_outside_dfsph(lib, solver):
+    """Only the DFSPH tick adapts dt and advances the simulated time by it; a
+    CFL flag on another solver would leave sim_time at 0 and vtp_run looping."""
+    cfg = vtp.config_template(solver)
+    dom = next(k for k in cfg if k != "EXPORT")
+    cfg[dom]["Extensions"] = {"cflEnable": True}
+    with pytest.raises(vtp.VtpError, match="cflEnable exists for DFSPH only"):
+        vtp.params_from_json(solver, json.dumps(cfg))
+    p = vtp.params_default(solver)
+    p.cfl_enable = 1
+    h = C.c_void_p()
+    assert lib.vtp_create_with_params(C.byref(h), C.byref(p)) == _abi.VTP_EINVAL
+    assert h.value is None
+    assert b"DFSPH only" in lib.vtp_last_error(None)
+
+
 def test_state_header_layout_matches_c(tmp_path):
     """ctypes VtpStateHeader has the C struct's size and field offsets."""
     import subprocess

@@ -242,3 +242,26 @@ def test_taped_last_segment_and_repeated_backward():
     for r in (second, whole, whole_again):
         assert np.array_equal(first[0], r[0]) and np.array_equal(first[1], r[1]) and first[2] == r[2]
     ctx.close()
+
+
+def test_adjoint_refuses_rollouts_with_capped_lists():
+    """The reverse passes need symmetric lists (j in N(i) <=> i in N(j)),
+    which a list cut by maxNeighborNum breaks.  A clump whose lists reach a
+    cap of 20 (33 neighbours at rest) is refused by the forward and the
+    backward with VTP_ENBRCAP; the same clump with the template cap of 60
+    differentiates normally.  The context itself keeps working."""
+    wl = scene()
+    wl.config["PBF"]["Required"]["maxNeighborNum"] = 20
+    p, ctx = setup(wl)
+    with pytest.raises(vtp.VtpError, match="VTP_ENBRCAP"):
+        ctx.rollout_fwd(4, 2)
+    with pytest.raises(vtp.VtpError, match="VTP_ENBRCAP"):
+        ctx.rollout_bwd(np.ones((wl.n, 3), np.float32), None)
+    ctx.step(1)   # plain ticks are unaffected
+    ctx.close()
+    ok = scene()
+    p, ctx = setup(ok)
+    ctx.rollout_fwd(4, 2)
+    gv0, _, _ = ctx.rollout_bwd(np.ones((ok.n, 3), np.float32), None)
+    assert np.isfinite(gv0).all() and np.abs(gv0).max() > 0
+    ctx.close()

@@ -187,3 +187,95 @@ def test_ugns_pairs_at_the_cutoff_across_cell_faces_and_corners(walk):
     g, st = _compare_ugns(wl, walk=walk)
     assert st["dead_particles"] == 0
     assert g["neighbor_num"].max() < 256
+
+
+def _per_tick_workload(solver):
+    """A jittered 12^3 block in its box, four isolated particles just outside
+    the box's +z walls (still on the grid) and two particles off the grid
+    from the start.  In the DFSPH run the four move out at 5, 8, 12 and
+    16 m/s and have left the grid by the searches of ticks 4, 3, 3 and 2; the
+    PBF run zeroes the velocities before every tick, so there they stay."""
+    if solver == "PBF":
+        cfg = scenes.pbf_config(lb=(-0.1, -0.1, -0.1), size=(0.6, 0.6, 0.6))
+        cfg["PBF"]["Optional"] = {"enable": True, "gravity": [0.0, 0.0, 0.0]}
+        spacing, zfly = 0.02, 0.485
+    else:
+        cfg = scenes.dfsph_config(lb=(-0.05, -0.05, -0.05), size=(0.3, 0.3, 0.3))
+        spacing, zfly = 0.01, 0.2425
+    objs = scenes.dam_break("tick", solver, cfg, (12, 12, 12), spacing, (spacing * 20,) * 3,
+                            (spacing * 3,) * 3, seed=4)
+    fly = np.array([[x, 0.5, zfly / (spacing * 20)] for x in (0.15, 0.35, 0.55, 0.75)], np.float32) * spacing * 20
+    fly_v = np.zeros((4, 3), np.float32)
+    if solver == "DFSPH":
+        fly_v[:, 2] = [5.0, 8.0, 12.0, 16.0]
+    gone = np.array([[50.0, 0.0, 0.0], [-50.0, 0.1, 0.1]], np.float32)
+    objs.append(scenes.Obj(vtp.MAT_FLUID, fly, fly_v))
+    objs.append(scenes.Obj(vtp.MAT_FLUID, gone))
+    return scenes.Workload("per_tick", solver, cfg, objs)
+
+
+@pytest.mark.parametrize("solver", ["PBF", "DFSPH"])
+@pytest.mark.parametrize("walk,radix", [(1, 0), (2, 0), (1, 1), (2, 1)])
+def test_ugns_per_tick_search_bit_exact(solver, walk, radix):
+    """The search every tick runs (UGNSCudaImpl.cu:205-227) starts from the
+    previous tick's slot order and restores the (cellId, attach id) order in
+    the cells (k_cell_fixup); its lists are built after the state is
+    reordered.  Before each of four ticks the block is shifted by
+    +-(0.3, 0.2, 0.1) h, so a third of it changes cells every tick; after the
+    tick the device's sorted ids, cell tables and neighbour lists (reference
+    layout) equal the oracle's search of the same FP32 positions: the
+    positions the tick searched.
+
+    DFSPH searches the positions it starts the tick with.  PBF searches its
+    predicted positions; with gravity off and the velocities zeroed before a
+    tick, x + dt (v + dt g) is x bit for bit, so the written positions are
+    the searched ones.  Off-grid particles sit after the live ones; the
+    reference leaves their order to stale data (A10), so they are compared as
+    a set.
+    """
+    wl = _per_tick_workload(solver)
+    params = vtp.params_from_json(solver, wl.config_json())
+    ctx = vtp.Context(params)
+    ctx.set_option("neighbor_walk", walk)
+    ctx.set_option("radix_sort", radix)
+    if solver == "PBF":
+        ctx.set_option("static_lists", 1)
+    for o in wl.objects:
+        ctx.add_particles(o.pos, o.mat, o.vel, o.vol_frac, o.scene)
+    ctx.initialize()
+    _, cells = ctx.grid_info()
+    h = 4 * params.particle_radius
+    block = np.arange(wl.objects[0].pos.shape[0])
+    orc = pyoracle.Oracle.from_workload(params, wl)
+    deads = []
+    moved = []
+    prev = None
+    for t, sign in enumerate((1, 1, -1, -1)):
+        pos = ctx.read("pos")
+        pos[block] += (sign * np.array([0.3, 0.2, 0.1]) * h).astype(np.float32)
+        ctx.write("pos", pos)
+        if solver == "PBF":
+            ctx.write("vel", np.zeros((wl.n, 3), np.float32))
+        print(f"per-tick search {solver} walk={walk} radix={radix} tick {t + 1}", flush=True)
+        ctx.step(1)
+        g = ctx.debug_neighbors(1)
+        orc.write(0, pos.astype(np.float64))
+        orc.search()
+        r = orc.neighbors(cells)
+        dead = int((r["neighbor_num"] == 0).sum())   # a live particle lists itself
+        assert ctx.stats()["dead_particles"] == dead
+        deads.append(dead)
+        live = wl.n - dead
+        np.testing.assert_array_equal(g["sorted"][:live], r["sorted"][:live])
+        assert set(g["sorted"][live:].tolist()) == set(r["sorted"][live:].tolist())
+        np.testing.assert_array_equal(g["cell_start"], r["cell_start"])
+        np.testing.assert_array_equal(g["cell_end"], r["cell_end"])
+        np.testing.assert_array_equal(g["neighbor_num"], r["neighbor_num"])
+        np.testing.assert_array_equal(g["neighbors"], r["neighbors"])
+        if prev is not None:
+            moved.append(int((g["sorted"] != prev).sum()))
+        prev = g["sorted"].copy()
+    # slots changed between ticks; particles left the grid during the DFSPH run
+    assert min(moved) > wl.n // 10, moved
+    assert deads == ([2, 2, 2, 2] if solver == "PBF" else [2, 3, 5, 6]), deads
+    ctx.close()

@@ -15,7 +15,9 @@
 //   predict    v1 = v0 + g dt, x1 = x0 + v1 dt
 // with neighbour lists held fixed, H_ij = f I + (f'(r)/r) r r^T the Hessian
 // of W, and symmetric lists (j in N(i) <=> i in N(j)), which holds whenever
-// no list is cut by maxNeighborNum.
+// no list is cut by maxNeighborNum.  The forward counts the particles whose
+// list reached the cap in every build of the rollout (kStCapSeen); when any
+// did, the rollout is refused with VTP_ENBRCAP and no gradient is returned.
 #include <cstring>
 #include <memory>
 
@@ -39,6 +41,10 @@ __device__ __forceinline__ void kernel_f_d(float r2, const Cubic &k, float &f, f
     const bool ok = r2 >= cubic_r2_lo(k);
     f = ok ? G * rinv : 0.0f;
     d = ok ? rinv * rinv * (dG - G * rinv) : 0.0f;
+}
+
+__global__ void k_adj_clear_word(uint32_t *w) {
+    if (threadIdx.x == 0) *w = 0u;
 }
 
 __global__ void k_adj_mat_from_orig(const uint32_t *__restrict__ orig, const int32_t *__restrict__ mat_attach,
@@ -361,6 +367,7 @@ struct AdjointState {
     // backward reverses needs no re-forward.  A backward that re-forwards an
     // earlier segment overwrites those tapes.
     bool tail_taped = false;
+    uint32_t capped = 0;   // particles at the cap summed over the forward's builds (0 = lists symmetric)
     void drop_graphs() {
         for (auto &e : seg_exec)
             if (e) (void)hipGraphExecDestroy(e);
@@ -554,6 +561,9 @@ extern "C" vtp_status vtp_pbf_rollout_fwd(vtp_ctx *ctx, uint32_t nsteps, uint32_
         }
         ensure_tapes(c, A);
         A.tail_taped = false;
+        A.capped = 0;
+        // the builds of the rollout add their cap counts to this word
+        hipLaunchKernelGGL(k_adj_clear_word, dim3(1), dim3(64), 0, c.stream, c.status.p + kStCapSeen);
         save_ckpt(c, *A.ckpt[0]);
         const uint32_t tail0 = (A.nseg - 1) * ckpt_every;   // first tick of the last segment
         for (uint32_t t = 0; t < nsteps; ++t) {
@@ -577,6 +587,15 @@ extern "C" vtp_status vtp_pbf_rollout_fwd(vtp_ctx *ctx, uint32_t nsteps, uint32_
             c.err = "solver crashed during the rollout";
             return VTP_ECRASH;
         }
+        A.capped = c.h_status[kStCapSeen];
+        if (A.capped) {
+            // a list cut by the cap breaks j in N(i) <=> i in N(j), which the
+            // reverse passes rely on: no gradient for this rollout
+            c.err = "a neighbour list reached maxNeighborNum during the rollout (" + std::to_string(A.capped) +
+                    " particle-builds at the cap): the lists are not symmetric, so the adjoint would be wrong; "
+                    "raise maxNeighborNum";
+            return VTP_ENBRCAP;
+        }
         A.valid = true;
         A.tail_taped = true;
         return VTP_OK;
@@ -590,6 +609,10 @@ extern "C" vtp_status vtp_pbf_rollout_bwd(vtp_ctx *ctx, const float *grad_x, con
     vtp_status s = adj_check(c);
     if (s != VTP_OK) return s;
     AdjointState &A = adj(c);
+    if (A.capped) {
+        c.err = "the last rollout had neighbour lists at maxNeighborNum: no gradient (raise maxNeighborNum)";
+        return VTP_ENBRCAP;
+    }
     if (!A.valid) { c.err = "vtp_pbf_rollout_bwd needs a preceding vtp_pbf_rollout_fwd"; return VTP_ESTATE; }
     return adj_guarded(c, [&]() -> vtp_status {
         const uint32_t n = c.n;

@@ -197,6 +197,9 @@ extern "C" vtp_status vtp_params_from_json(int32_t solver, const char *json_text
                 for (uint32_t k = 0; k < p.phase_num; ++k) p.phase_rest_viscosity[k] = vis[k];
             }
         }
+        if (p.cfl_enable && solver != VTP_DFSPH)
+            throw std::runtime_error(dom + " Extensions.cflEnable exists for DFSPH only (the other solvers step "
+                                           "with the fixed timeStep)");
         if (p.particle_radius <= 0.f || p.time_step <= 0.f)
             throw std::runtime_error(dom + " particleRadius and timeStep must be positive");
         if (p.max_neighbor_num < 1) throw std::runtime_error(dom + " maxNeighborNum must be >= 1");

@@ -223,6 +223,11 @@ extern "C" vtp_status vtp_create_with_params(vtp_ctx **out, const vtp_params *pa
         g_create_error = "vtp_create: unknown solver type";
         return VTP_EINVAL;
     }
+    if (params->cfl_enable && params->solver != VTP_DFSPH) {
+        // only the DFSPH tick adapts dt and advances the simulated time by it
+        g_create_error = "vtp_create: cfl_enable exists for DFSPH only";
+        return VTP_EINVAL;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
         g_create_error = "vtp_create: no HIP device available";
@@ -360,7 +365,8 @@ extern "C" vtp_status vtp_step(vtp_ctx *ctx, uint32_t nsteps) {
         for (uint32_t s = 0; s < nsteps; ++s) {
             solver_tick(c);
             c.steps += 1;
-            if (c.p.solver == VTP_PBF || !c.p.cfl_enable) c.sim_time += c.p.time_step;
+            // a DFSPH tick with CFL advances sim_time by its own dt (dfsph_step)
+            if (!(c.p.solver == VTP_DFSPH && c.p.cfl_enable)) c.sim_time += c.p.time_step;
         }
         return check_after_steps(c);
     });
@@ -480,7 +486,8 @@ extern "C" vtp_status vtp_read(vtp_ctx *ctx, int32_t field, void *host_dst, size
             case VTP_FIELD_COMPRESSION_RATIO: f1(c.cr.p); break;
             case VTP_FIELD_DF_ALPHA: f1(c.alpha.p); break;
             case VTP_FIELD_DELTA_CR: f1(c.delta.p); break;
-            case VTP_FIELD_KAPPA_DIV: case VTP_FIELD_KAPPA_INCOMP: f1(c.kappa.p); break;
+            case VTP_FIELD_KAPPA_DIV: f1(c.kappa_div.p); break;
+            case VTP_FIELD_KAPPA_INCOMP: f1(c.kappa.p); break;
             case VTP_FIELD_VEL_ADV: f4_xyz(c.vadv.p); break;
             case VTP_FIELD_ACC: f4_xyz(c.acc.p); break;
             case VTP_FIELD_SURFACE_NORMAL: f4_xyz(c.normal.p); break;

@@ -484,6 +484,9 @@ void df_pressure_solve(Ctx &c, const float4 *x, const float4 *v, int which) {
     const char *name_dk = which == 0 ? "df_div_delta_kappa" : "df_inc_delta_kappa";
     const char *name_up = which == 0 ? "df_div_update_vadv" : "df_inc_update_vadv";
     const uint32_t bd = c.block("df_delta_kappa"), bu = c.block("df_update_vadv");
+    // kappa_div and kappa_incomp are separate arrays, as in the reference
+    // (DFSPHCudaImpl.cu:182-196 and 355-367), so both stay readable
+    float *kap = which == 0 ? c.kappa_div.p : c.kappa.p;
     hipLaunchKernelGGL(k_solve_begin, dim3(1), dim3(256), 0, c.stream, c.status.p, c.scal.p, lp.nobj);
     uint32_t &hint = which == 0 ? c.iter_hint_div : c.iter_hint_inc;
     uint32_t chunk = c.opt_chunk ? c.opt_chunk : std::max(1u, hint);
@@ -493,11 +496,11 @@ void df_pressure_solve(Ctx &c, const float4 *x, const float4 *v, int which) {
             c.timer.begin(c.stream, name_dk);
             hipLaunchKernelGGL(k_df_delta_kappa, dim3(div_up(n, bd)), dim3(bd), 0, c.stream, x, v, c.vadv.p, c.mat.p,
                                c.obj_of.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, c.cr.p, c.alpha.p, c.delta.p,
-                               c.kappa.p, c.xk.p, lp, launched == 0 ? 1 : 0);
+                               kap, c.xk.p, lp, launched == 0 ? 1 : 0);
             c.timer.end(c.stream);
             c.timer.begin(c.stream, name_up);
             hipLaunchKernelGGL(k_df_update_vadv, dim3(div_up(n, bu)), dim3(bu), 0, c.stream, x, v, c.xk.p, c.g1.p,
-                               c.vadv.p, c.mat.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, c.kappa.p, lp);
+                               c.vadv.p, c.mat.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, kap, lp);
             c.timer.end(c.stream);
             hipLaunchKernelGGL(k_df_decide, dim3(1), dim3(256), 0, c.stream, lp, div_up(n, bd), thr, cap, word,
                                bit, which);
@@ -520,6 +523,7 @@ void dfsph_allocate_common(Ctx &c) {
     c.cr.alloc(n);
     c.alpha.alloc(n);
     c.delta.alloc(n);
+    c.kappa_div.alloc(n);
     c.kappa.alloc(n);
     c.vadv.alloc(n);
     c.acc.alloc(n);
@@ -529,6 +533,7 @@ void dfsph_allocate_common(Ctx &c) {
     VTP_HIP(hipMemsetAsync(c.cr.p, 0, c.cr.bytes(), c.stream));
     VTP_HIP(hipMemsetAsync(c.alpha.p, 0, c.alpha.bytes(), c.stream));
     VTP_HIP(hipMemsetAsync(c.delta.p, 0, c.delta.bytes(), c.stream));
+    VTP_HIP(hipMemsetAsync(c.kappa_div.p, 0, c.kappa_div.bytes(), c.stream));
     VTP_HIP(hipMemsetAsync(c.kappa.p, 0, c.kappa.bytes(), c.stream));
     VTP_HIP(hipMemsetAsync(c.acc.p, 0, c.acc.bytes(), c.stream));
     VTP_HIP(hipMemsetAsync(c.normal.p, 0, c.normal.bytes(), c.stream));

@@ -784,8 +784,10 @@ static void imm_init_impl(Ctx &c) {
     hipLaunchKernelGGL(k_imm_init_a<MP>, dim3(nb), dim3(256), 0, c.stream, c.X[0].p, c.V[0].p, c.vadv.p,
                        c.rest2[0].p, c.cd.p, c.vis_ct.p, c.Q[0].p, c.color4[0].p, c.mat.p, n, q, ph);
     // prepare_imm: rest density (boundary max) and mass with V = V0, then boundary volume
-    VTP_HIP(hipMemcpyAsync(c.kappa.p, c.rest2[0].p, n * 4, hipMemcpyDeviceToDevice, c.stream));
-    hipLaunchKernelGGL(k_imm_boundary_rest, dim3(nb), dim3(256), 0, c.stream, c.X[0].p, c.V[0].p, c.kappa.p,
+    // the boundary maximum reads a copy of the rest densities (race-free, A3);
+    // rest2[1] is the unused half of the rest-density ping-pong here
+    VTP_HIP(hipMemcpyAsync(c.rest2[1].p, c.rest2[0].p, n * 4, hipMemcpyDeviceToDevice, c.stream));
+    hipLaunchKernelGGL(k_imm_boundary_rest, dim3(nb), dim3(256), 0, c.stream, c.X[0].p, c.V[0].p, c.rest2[1].p,
                        c.rest2[0].p, c.mat.p, c.ugns.nbr.p, c.ugns.cnt.p, n, c.ugns.stride, c.kern, 1);
     VTP_HIP(hipGetLastError());
 }
@@ -928,8 +930,9 @@ static void imm_step_impl(Ctx &c) {
     c.timer.begin(c.stream, "imm_phase_update");
     hipLaunchKernelGGL(k_imm_phase_update<MP>, dim3(nb), dim3(256), 0, c.stream, x, v, c.vadv.p, c.mat.p, c.flag.p,
                        rest, c.color4[0].p, n, q, ph);
-    VTP_HIP(hipMemcpyAsync(c.kappa.p, rest, n * 4, hipMemcpyDeviceToDevice, c.stream));
-    hipLaunchKernelGGL(k_imm_boundary_rest, dim3(nb), dim3(256), 0, c.stream, x, v, c.kappa.p, rest, c.mat.p,
+    // rest2[1] holds the previous slot order's rest densities, no longer read
+    VTP_HIP(hipMemcpyAsync(c.rest2[1].p, rest, n * 4, hipMemcpyDeviceToDevice, c.stream));
+    hipLaunchKernelGGL(k_imm_boundary_rest, dim3(nb), dim3(256), 0, c.stream, x, v, c.rest2[1].p, rest, c.mat.p,
                        u.nbr.p, u.cnt.p, n, st, c.kern, 0);
     c.timer.end(c.stream);
     VTP_HIP(hipGetLastError());

@@ -632,7 +632,10 @@ __global__ __launch_bounds__(kNbrBuildThreads, VTP_BUILD_MIN_WAVES) void k_build
     }
     // particles that reached the cap steer the walk of the next build
     const uint64_t m = __ballot(full);
-    if (m && (threadIdx.x & 63u) == 0) atomicAdd(&status[kStNbrAtCap], (uint32_t)__popcll(m));
+    if (m && (threadIdx.x & 63u) == 0) {
+        atomicAdd(&status[kStNbrAtCap], (uint32_t)__popcll(m));
+        atomicAdd(&status[kStCapSeen], (uint32_t)__popcll(m));
+    }
     if (kStaged) {
         // every lane reads back only its own LDS column: no barrier needed
         const uint32_t cmax = min(wave_max_u32(c), stage);

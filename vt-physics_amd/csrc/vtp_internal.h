@@ -149,6 +149,8 @@ enum StatusWord : uint32_t {
     kStNegCount = 7,     // newly flagged negative fractions (IMM)
     kStNbrAtCap = 8,     // particles whose list reached the cap in the running build
     kStNbrPrev = 9,      // the same count of the previous build (steers the walk)
+    kStCapSeen = 10,     // particles at the cap summed over builds since the host cleared it
+                         // (the adjoint's rollout checks that no list was cut)
     kStWords = 16
 };
 
@@ -325,7 +327,8 @@ struct Ctx {
     bool lists_stale = false;   // neighbour lists do not match X[0] (after a rollout backward)
 
     // DFSPH / IMM fields (slot order)
-    DevBuf<float> rest_density, vis, cr, alpha, delta, kappa;
+    DevBuf<float> rest_density, vis, cr, alpha, delta;
+    DevBuf<float> kappa_div, kappa;          // kappa of the divergence solve / of the density solve
     DevBuf<float> rest_attach, vis_attach;   // static per-particle properties, attach order
     DevBuf<uint32_t> obj_cnt;                // particles per attach object
     DevBuf<int32_t> obj_fluid;               // 1 if the object is fluid

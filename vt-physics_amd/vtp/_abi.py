@@ -16,9 +16,9 @@ import numpy as np
 PKG_DIR = Path(__file__).resolve().parent.parent
 LIB_PATH = PKG_DIR / "libvtp.so"
 
-VTP_OK, VTP_EINVAL, VTP_ENOMEM, VTP_EHIP, VTP_ECRASH, VTP_ESTATE, VTP_ENODEV = range(7)
+VTP_OK, VTP_EINVAL, VTP_ENOMEM, VTP_EHIP, VTP_ECRASH, VTP_ESTATE, VTP_ENODEV, VTP_ENBRCAP = range(8)
 STATUS_NAMES = {0: "VTP_OK", 1: "VTP_EINVAL", 2: "VTP_ENOMEM", 3: "VTP_EHIP", 4: "VTP_ECRASH",
-                5: "VTP_ESTATE", 6: "VTP_ENODEV"}
+                5: "VTP_ESTATE", 6: "VTP_ENODEV", 7: "VTP_ENBRCAP"}
 PBF, DFSPH, IMM, IMMCT = 0, 1, 2, 3
 SOLVER_IDS = {"PBF": PBF, "DFSPH": DFSPH, "IMM": IMM, "IMMCT": IMMCT}
 MAT_FLUID, MAT_BOUNDARY = 0, 1

@@ -432,3 +432,28 @@ def cfg4_adjoint(seed: int = 0, n_side=(64, 32, 32)) -> Workload:
 
 WORKLOADS = {"cfg0": cfg0_pbf, "pbf1m": pbf_1m, "cfg1": cfg1_dfsph, "dfsph1m": dfsph_1m, "cfg2": cfg2_immct,
              "cfg3": cfg3_batched, "cfg4": cfg4_adjoint}
+
+
+def from_scene_document(doc: dict, name: str = "scene") -> Workload:
+    """A reference scene file (solver config + OBJECTS) as a Workload.
+
+    Follows ObjectManager::createObjectsFromJson (ObjectManager.cpp:22-41)
+    and the attach step of the solvers (PBFSolver.cpp:169-221): each object
+    is generated from its ObjectComponentConfig (objTransferType selects the
+    generator), gets the material epmMaterial, the velocity velocityStart and,
+    for IMM / IMM-CT, the fractions phaseFraction of its
+    SolverObjectComponentConfig, and is attached in file order.
+    """
+    solver = next(k for k in doc if k not in ("EXPORT", "OBJECTS"))
+    config = {k: v for k, v in doc.items() if k != "OBJECTS"}
+    objs = []
+    for o in doc.get("OBJECTS", []):
+        comp = dict(o["ObjectComponentConfig"], genType=o["objTransferType"])
+        pos = generate(comp).astype(np.float32)
+        sc = o["SolverObjectComponentConfig"]
+        vel = np.tile(np.asarray(sc.get("velocityStart", [0, 0, 0]), np.float32), (pos.shape[0], 1))
+        frac = None
+        if "phaseFraction" in sc:
+            frac = np.tile(np.asarray(sc["phaseFraction"], np.float32), (pos.shape[0], 1))
+        objs.append(Obj(int(comp["epmMaterial"]), pos, vel, frac, 0))
+    return Workload(name, solver, config, objs, steps=1)
