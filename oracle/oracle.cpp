@@ -172,6 +172,9 @@ struct Sim {
     uint32_t div_it = 0, inc_it = 0, phase_it = 0, crash = 0;
     double div_res = 0, inc_res = 0;
     bool initialized = false;
+    // test probe: 1 = a DFSPH tick returns right after its divergence solve
+    // (v* of that solve left in vadv, before the pressure is applied)
+    int stop_after = 0;
 };
 
 template <typename F>
@@ -520,6 +523,7 @@ void df_tick(Sim &s) {
     ugns_update(s);
     df_precompute(s);
     df_pressure_solve(s, s.p.div_free_threshold, s.kdiv, s.div_it, s.div_res, VTP_CRASH_DIV_ITER_CAP);
+    if (s.stop_after == 1) return;
     df_apply_pressure(s);
     // dfsph_gravity_vis_surface :523-554
     const double h2_001 = 0.001 * s.h * s.h;
@@ -706,13 +710,32 @@ extern "C" int orc_read(orc_sim *o, int32_t field, double *out) {
             for (uint32_t i = 0; i < n; ++i) for (int a = 0; a < 9; ++a) out[9 * i + a] = m[i].m[a];
             return 0;
         }
+        // probes of the oracle's internal IMM-CT arrays (no device counterpart)
+        case ORC_PROBE_BETA: return put1(out, s.beta_ct, n);
+        case ORC_PROBE_KAPPA_CT: return put1(out, s.kappa_ct, n);
+        case ORC_PROBE_BLOCKING: return put1(out, s.blocking, n);
         default: return -3;
     }
+}
+
+extern "C" void orc_set_stop(orc_sim *o, int stage) {
+    if (o) o->s.stop_after = stage;
 }
 
 extern "C" int orc_write(orc_sim *o, int32_t field, const double *src) {
     if (!o || !src) return -1;
     Sim &s = o->s;
+    if (field == VTP_FIELD_VEL_PHASE) {   // [n][P][3], after initialize
+        if (s.uph.size() != (size_t)s.n * s.P) return -2;
+        for (size_t i = 0; i < s.uph.size(); ++i) s.uph[i] = {src[3 * i], src[3 * i + 1], src[3 * i + 2]};
+        return 0;
+    }
+    if (field == VTP_FIELD_Q) {           // [n][9] row-major, after initialize (IMM-CT)
+        if (s.Q.size() != s.n) return -2;
+        for (uint32_t i = 0; i < s.n; ++i)
+            for (int a = 0; a < 9; ++a) s.Q[i].m[a] = src[9 * i + a];
+        return 0;
+    }
     std::vector<V3> *dst = field == VTP_FIELD_POS ? &s.pos : field == VTP_FIELD_VEL ? &s.vel : nullptr;
     if (!dst) return -3;
     for (uint32_t i = 0; i < s.n; ++i) (*dst)[i] = {src[3 * i], src[3 * i + 1], src[3 * i + 2]};

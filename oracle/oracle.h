@@ -37,7 +37,13 @@ int orc_step(orc_sim *s, uint32_t nsteps);
 /* Field ids are vtp_field values; output in attach order as doubles, with
  * the same element counts as vtp_read. */
 int orc_read(orc_sim *s, int32_t field, double *out);
+/* POS, VEL (any time), VEL_PHASE [n][P][3] and Q [n][9] (after initialize). */
 int orc_write(orc_sim *s, int32_t field, const double *src);
+/* Test probes: orc_read of the IMM-CT rheological factors beta, kappa and
+ * the polymer blocking sum of the last tick (one double per particle); and
+ * orc_set_stop(1): a DFSPH tick returns right after its divergence solve. */
+enum { ORC_PROBE_BETA = 100, ORC_PROBE_KAPPA_CT = 101, ORC_PROBE_BLOCKING = 102 };
+void orc_set_stop(orc_sim *s, int stage);
 /* Reference-layout UGNS state of the last search (sizes N*M, N, N, C, C). */
 int orc_neighbors(orc_sim *s, uint32_t *lists, uint32_t *num, uint32_t *sorted, uint32_t *cell_start,
                   uint32_t *cell_end);

@@ -34,6 +34,8 @@ def load():
         lib.orc_write.argtypes = [C.c_void_p, C.c_int32, _D]
         lib.orc_neighbors.argtypes = [C.c_void_p, _U, _U, _U, _U, _U]
         lib.orc_search.argtypes = [C.c_void_p]
+        lib.orc_set_stop.argtypes = [C.c_void_p, C.c_int]
+        lib.orc_set_stop.restype = None
         lib.orc_stats.argtypes = [C.c_void_p, _U, _U, _U, _D, _D, _U]
         lib.orc_destroy.argtypes = [C.c_void_p]
         lib.orc_destroy.restype = None
@@ -63,6 +65,8 @@ def _up(a):
 
 
 FIELD_COMPS = {0: 3, 1: 3, 14: 3, 15: 3, 16: 3, 20: 3, 22: 9, 23: 9, 24: 9}
+# probes of internal IMM-CT arrays (oracle.h)
+PROBE_BETA, PROBE_KAPPA_CT, PROBE_BLOCKING = 100, 101, 102
 
 
 class Oracle:
@@ -133,6 +137,10 @@ class Oracle:
         self.lib.orc_neighbors(self.h, _up(lists), _up(num), _up(srt), _up(cs), _up(ce))
         return {"neighbors": lists.reshape(self.n, M), "neighbor_num": num, "sorted": srt,
                 "cell_start": cs, "cell_end": ce}
+
+    def set_stop(self, stage: int):
+        """Test probe: 1 = a DFSPH tick returns right after its divergence solve."""
+        self.lib.orc_set_stop(self.h, stage)
 
     def search(self):
         """Neighbour search of the current positions alone (no step)."""
