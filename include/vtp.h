@@ -348,6 +348,18 @@ vtp_status vtp_pbf_rollout_bwd(vtp_ctx *ctx, const float *grad_x, const float *g
  * the last neighbour lists, L = mean_f (rho_f/rho0 - 1)^2 (unclamped SPH
  * density), and dL/dx (float[n][3], attach order; may be NULL). */
 vtp_status vtp_pbf_density_loss(vtp_ctx *ctx, double *loss, float *grad_x);
+/* vtp_pbf_density_loss without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under hipGraph
+ * capture of their own, tight loops, harnesses that forbid hipMalloc).
+ * Outputs are DEVICE pointers on the context's GPU: d_loss one double,
+ * d_grad_x float[n][3] in attach order (may be NULL).  Its scratch (the loss
+ * accumulator and dL/drho per particle) is a fixed array of 16 MiB + 256 B on
+ * each GPU the library runs on, held until the process ends, so contexts of
+ * at most 4,194,304 particles are accepted (VTP_EINVAL above that).  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (from
+ * several host threads or contexts) race.  Returns after the context's stream
+ * has finished the work. */
+vtp_status vtp_pbf_density_loss_noalloc(vtp_ctx *ctx, double *d_loss, float *d_grad_x);
 /* Sets XSPH_k (the learnable coefficient) on an existing context. */
 vtp_status vtp_pbf_set_xsph(vtp_ctx *ctx, float xsph_k);
 

@@ -20,6 +20,7 @@
 // did, the rollout is refused with VTP_ENBRCAP and no gradient is returned.
 #include <cstring>
 #include <memory>
+#include <mutex>
 
 #include "vtp_internal.h"
 
@@ -312,9 +313,11 @@ __global__ void k_dl_rho(const float4 *__restrict__ X, const int32_t *__restrict
     if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(loss, s);
 }
 
+// g: slot order (float4), or, when ga is set, ga[orig[p]] in attach order (float[3]).
 __global__ void k_dl_grad(const float4 *__restrict__ X, const float *__restrict__ rb, const int32_t *__restrict__ mat,
                           const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ cnt, uint32_t n,
-                          uint32_t stride, PbfK q, float4 *__restrict__ g) {
+                          uint32_t stride, PbfK q, float4 *__restrict__ g, float *__restrict__ ga,
+                          const uint32_t *__restrict__ orig) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     float ax = 0, ay = 0, az = 0;
@@ -333,7 +336,23 @@ __global__ void k_dl_grad(const float4 *__restrict__ X, const float *__restrict_
                 az = fmaf(dz, f, az);
             });
     }
-    g[p] = make_float4(ax, ay, az, 0.0f);
+    if (ga) {
+        float *o = ga + 3 * (size_t)orig[p];
+        o[0] = ax;
+        o[1] = ay;
+        o[2] = az;
+    } else {
+        g[p] = make_float4(ax, ay, az, 0.0f);
+    }
+}
+
+__global__ void k_dl_clear(double *acc) {
+    if (threadIdx.x == 0) *acc = 0.0;
+}
+
+// loss = sum / fluid count, in FP64 as the host does it for vtp_pbf_density_loss
+__global__ void k_dl_finish(const double *__restrict__ acc, double nf, double *__restrict__ loss) {
+    if (threadIdx.x == 0) *loss = nf > 0.0 ? *acc / nf : 0.0;
 }
 
 // ------------------------------------------------------------------ host side
@@ -729,7 +748,7 @@ extern "C" vtp_status vtp_pbf_density_loss(vtp_ctx *ctx, double *loss, float *gr
         hipLaunchKernelGGL(k_dl_rho, dim3(nb), dim3(256), 0, c.stream, c.X[0].p, c.mat.p, u.nbr.p, u.cnt.p, n,
                            u.stride, q, nf ? 1.0f / nf : 0.0f, rb.p, L.p);
         hipLaunchKernelGGL(k_dl_grad, dim3(nb), dim3(256), 0, c.stream, c.X[0].p, rb.p, c.mat.p, u.nbr.p, u.cnt.p, n,
-                           u.stride, q, g.p);
+                           u.stride, q, g.p, nullptr, nullptr);
         VTP_HIP(hipGetLastError());
         double lsum = 0;
         VTP_HIP(hipMemcpyAsync(&lsum, L.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -748,6 +767,77 @@ extern "C" vtp_status vtp_pbf_density_loss(vtp_ctx *ctx, double *loss, float *gr
                 grad_x[3 * a + 1] = hg[i].y;
                 grad_x[3 * a + 2] = hg[i].z;
             }
+        return VTP_OK;
+    });
+}
+
+// ---- allocation-free density loss -----------------------------------------
+// The scratch of vtp_pbf_density_loss (the loss accumulator and dL/drho per
+// slot) in a fixed device array, so that this variant calls no allocator.
+// Layout: [0, 256) the FP64 accumulator, then float rb[kNoAllocMaxParticles].
+// Each device holds its own copy at its own address, looked up once per
+// device; host code uses that address, never the array's name.
+constexpr uint32_t kNoAllocMaxParticles = 1u << 22;
+constexpr size_t kNoAllocScratchBytes = 256 + sizeof(float) * (size_t)kNoAllocMaxParticles;   // 16 MiB + 256 B
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// The device address of `scratch` on `device` (current), looked up on the first call there.
+void *scratch_on(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices) throw std::runtime_error("device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        VTP_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw std::runtime_error("no device address for the scratch array");
+        g_scratch_ptr[device] = ptr;
+    }
+    return g_scratch_ptr[device];
+}
+}  // namespace
+
+extern "C" vtp_status vtp_pbf_density_loss_noalloc(vtp_ctx *ctx, double *d_loss, float *d_grad_x) {
+    if (!ctx) return VTP_EINVAL;
+    Ctx &c = ctx->c;
+    if (c.p.solver != VTP_PBF) { c.err = "density loss exists for PBF only"; return VTP_EINVAL; }
+    if (!c.initialized) { c.err = "not initialized"; return VTP_ESTATE; }
+    if (c.lists_stale) { c.err = "neighbour lists are stale after a rollout backward; tick first"; return VTP_ESTATE; }
+    if (!d_loss) { c.err = "vtp_pbf_density_loss_noalloc: null loss pointer"; return VTP_EINVAL; }
+    if (c.n > kNoAllocMaxParticles) {
+        c.err = "vtp_pbf_density_loss_noalloc: " + std::to_string(c.n) + " particles exceed its scratch of " +
+                std::to_string(kNoAllocMaxParticles) + "; use vtp_pbf_density_loss";
+        return VTP_EINVAL;
+    }
+    return adj_guarded(c, [&]() -> vtp_status {
+        int cur = 0;
+        VTP_HIP(hipGetDevice(&cur));
+        if (cur != c.device) VTP_HIP(hipSetDevice(c.device));
+        struct Restore {
+            int dev, was;
+            ~Restore() { if (dev != was) (void)hipSetDevice(was); }
+        } restore{c.device, cur};
+        unsigned char *base = static_cast<unsigned char *>(scratch_on(c.device));
+        double *acc = reinterpret_cast<double *>(base);
+        float *rb = reinterpret_cast<float *>(base + 256);
+        const uint32_t n = c.n, nb = div_up(n, 256);
+        const PbfK q = pbf_constants(c);
+        uint32_t nf = 0;
+        for (int32_t m : c.h_mat) nf += m == VTP_MAT_FLUID;
+        Ugns &u = c.ugns;
+        // the accumulator keeps the last call's sum: cleared by a kernel
+        hipLaunchKernelGGL(k_dl_clear, dim3(1), dim3(64), 0, c.stream, acc);
+        hipLaunchKernelGGL(k_dl_rho, dim3(nb), dim3(256), 0, c.stream, c.X[0].p, c.mat.p, u.nbr.p, u.cnt.p, n,
+                           u.stride, q, nf ? 1.0f / nf : 0.0f, rb, acc);
+        hipLaunchKernelGGL(k_dl_finish, dim3(1), dim3(64), 0, c.stream, acc, (double)nf, d_loss);
+        if (d_grad_x)
+            hipLaunchKernelGGL(k_dl_grad, dim3(nb), dim3(256), 0, c.stream, c.X[0].p, rb, c.mat.p, u.nbr.p,
+                               u.cnt.p, n, u.stride, q, nullptr, d_grad_x, c.orig.p);
+        VTP_HIP(hipGetLastError());
+        VTP_HIP(hipStreamSynchronize(c.stream));
         return VTP_OK;
     });
 }

@@ -186,6 +186,7 @@ SIGNATURES = {
     "vtp_pbf_rollout_fwd": (C.c_int, [_P, C.c_uint32, C.c_uint32]),
     "vtp_pbf_rollout_bwd": (C.c_int, [_P, _F, _F, _F, _F, _F]),
     "vtp_pbf_density_loss": (C.c_int, [_P, C.POINTER(C.c_double), _F]),
+    "vtp_pbf_density_loss_noalloc": (C.c_int, [_P, C.c_void_p, C.c_void_p]),
     "vtp_pbf_set_xsph": (C.c_int, [_P, C.c_float]),
     "vtp_version": (C.c_char_p, []),
 }
