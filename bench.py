@@ -33,7 +33,27 @@ def pbf_step_bytes(n, nf, cells, K, passes):
 
 
 def kernel_bytes(name, n, nf):
-    """Algorithmic bytes per launch of the neighbour passes (own fields once)."""
+    """Algorithmic bytes per launch of the PBF neighbour passes, split from the
+    per-tick formula above: an iteration's 32 N + 16 N_f is the density+lambda
+    pass (x, m in -> lambda out: 16 N + 4 N_f) plus the correction pass
+    (x, lambda in -> x out: 16 N + 12 N_f); XSPH + apply is 28 N + 24 N_f.
+    Neighbour re-reads and lists are not credited.  The build is credited
+    with the first iteration's density+lambda pass it computes."""
+    if name == "pbf_lambda":
+        return 16 * n + 4 * nf
+    if name == "pbf_dx":
+        return 16 * n + 12 * nf
+    if name == "pbf_xsph":
+        return 28 * n + 24 * nf
+    if name == "ugns_neighbors":
+        return 16 * n + 4 * nf
+    return None
+
+
+def kernel_bytes_own_fields(name, n, nf):
+    """Bytes per launch counting every field the kernel itself reads and
+    writes once (material words, both halves of the (x, lambda) records,
+    velocities), the per-kernel count of the design notes."""
     if name == "pbf_lambda":      # x,m + mat in; lambda out (all); rho, C out (fluid)
         return 24 * n + 8 * nf
     if name == "pbf_dx":          # x,m + lambda + mat in (all); x out, v in/out (fluid)
@@ -43,6 +63,24 @@ def kernel_bytes(name, n, nf):
     if name == "ugns_neighbors":  # x + key in; count word out; with the tick's first lambda pass
         return 36 * n + 8 * nf    # mat in, lambda out (all), rho and C out (fluid)
     return None
+
+
+def dfsph_step_bytes(n, nf, cells, passes, it_div, it_inc):
+    """Algorithmic HBM bytes of one DFSPH tick: grid, the alpha + compression
+    ratio pass, per solver iteration the delta+kappa and v* passes, two
+    pressure applications, the non-pressure forces and the position update."""
+    return (nf * (88 + 16 * passes) + 8 * cells + 20 * n + 8 * nf + (it_div + it_inc) * (48 * n + 40 * nf)
+            + 2 * 48 * nf + 150 * nf + 36 * nf)
+
+
+def immct_step_bytes(n, nf, cells, passes, it_div, it_inc, P, phase_passes, ct_passes=2):
+    """The DFSPH tick's bytes plus P (4 + 4 + 4 + 12 + 12 + 12) bytes per
+    fluid particle for every pass that touches the phases (fractions in and
+    out, fluxes, phase velocities, drift and phase acceleration) and
+    3 x 36 bytes per fluid particle (Q, L, stress) for each of the two
+    conformation-tensor passes."""
+    return (dfsph_step_bytes(n, nf, cells, passes, it_div, it_inc) + phase_passes * P * 48 * nf
+            + ct_passes * 3 * 36 * nf)
 
 
 def build_ctx(vtp, wl, graph=True):
@@ -75,9 +113,14 @@ def stage_breakdown(ctx, steps):
     return {k: round(v[0] / steps, 4) for k, v in st.items()}
 
 
+def key_passes(cells):
+    return 3 if cells >= (1 << 16) else 2
+
+
 def bench_dfsph(torch, vtp, scenes, steps, warmup):
     wl = scenes.dfsph_1m(seed=0)
     ctx, params = build_ctx(vtp, wl, graph=False)
+    _, cells = ctx.grid_info()
     ctx.step(warmup)
     st0 = ctx.stats()
     el = timed_steps(torch, ctx, steps)
@@ -86,10 +129,13 @@ def bench_dfsph(torch, vtp, scenes, steps, warmup):
     ctx.close()
     it_div = (st["total_div_iterations"] - st0["total_div_iterations"]) / steps
     it_inc = (st["total_incomp_iterations"] - st0["total_incomp_iterations"]) / steps
+    ms = el * 1e3 / steps
+    sb = dfsph_step_bytes(wl.n, wl.n_fluid, cells, key_passes(cells), it_div, it_inc)
     return {"workload": wl.name, "fluid_particles": wl.n_fluid, "boundary_particles": wl.n - wl.n_fluid,
             "value": round(wl.n_fluid * steps / el, 1), "unit": "particle-steps/s",
-            "ms_per_step": round(el * 1e3 / steps, 4), "mean_div_iterations": it_div,
+            "ms_per_step": round(ms, 4), "mean_div_iterations": it_div,
             "mean_incomp_iterations": it_inc, "steps": steps, "warmup": warmup,
+            "step_algorithmic_bytes": int(sb), "step_hbm_frac": round(sb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
             "dead_particles": st["dead_particles"], "stages_ms_per_step": stages}
 
 
@@ -99,14 +145,25 @@ def bench_immct(torch, vtp, scenes, steps, warmup):
     # r = 0.005 within two ticks; the timed run uses the mass-weighted form
     wl.config["IMMCT"]["Extensions"]["massWeighted"] = True
     ctx, params = build_ctx(vtp, wl, graph=False)
+    _, cells = ctx.grid_info()
     ctx.step(warmup)
+    st0 = ctx.stats()
     el = timed_steps(torch, ctx, steps)
     st = ctx.stats()
     stages = stage_breakdown(ctx, 1)
     ctx.close()
+    it_div = (st["total_div_iterations"] - st0["total_div_iterations"]) / steps
+    it_inc = (st["total_incomp_iterations"] - st0["total_incomp_iterations"]) / steps
+    ms = el * 1e3 / steps
+    # phase passes: pressure -> phases (x2), viscosity, tension + phase velocity,
+    # stress, phase update, and one flux pass per negative-fraction round
+    phase_passes = 6 + max(1, st["phase_iterations"])
+    sb = immct_step_bytes(wl.n, wl.n_fluid, cells, key_passes(cells), it_div, it_inc, 3, phase_passes)
     return {"workload": wl.name, "fluid_particles": wl.n_fluid, "boundary_particles": wl.n - wl.n_fluid,
             "stages_ms_per_step": stages, "phases": 3, "value": round(wl.n_fluid * steps / el, 1), "unit": "particle-steps/s",
-            "ms_per_step": round(el * 1e3 / steps, 4), "steps": steps, "warmup": warmup,
+            "ms_per_step": round(ms, 4), "steps": steps, "warmup": warmup,
+            "mean_div_iterations": it_div, "mean_incomp_iterations": it_inc,
+            "step_algorithmic_bytes": int(sb), "step_hbm_frac": round(sb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
             "mass_weighted_stress": True, "last_iterations": [st["div_iterations"], st["incomp_iterations"],
                                                               st["phase_iterations"]]}
 
@@ -130,13 +187,21 @@ def bench_batched(torch, vtp, scenes, steps, warmup):
         c.initialize()
         c.step(warmup)
         ctxs.append(c)
+    # each scene's ticks back to back with one synchronisation per scene (as
+    # the batched run has one for all of them)
     torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for c in ctxs:
+        c.step(steps)
+    torch.cuda.synchronize()
+    t_single = time.perf_counter() - t0
+    # the same, one tick of every scene in turn (a synchronisation per scene tick)
     t0 = time.perf_counter()
     for _ in range(steps):
         for c in ctxs:
             c.step(1)
     torch.cuda.synchronize()
-    t_single = time.perf_counter() - t0
+    t_inter = time.perf_counter() - t0
     for c in ctxs:
         c.close()
     return {"workload": wl.name, "scenes": len(per), "fluid_particles": wl.n_fluid,
@@ -144,6 +209,8 @@ def bench_batched(torch, vtp, scenes, steps, warmup):
             "batched_ms_per_step": round(t_batched * 1e3 / steps, 4),
             "one_context_per_scene_ms_per_step": round(t_single * 1e3 / steps, 4),
             "gain": round(t_single / t_batched, 3),
+            "one_context_per_scene_interleaved_ms_per_step": round(t_inter * 1e3 / steps, 4),
+            "gain_interleaved": round(t_inter / t_batched, 3),
             "batched_value": round(wl.n_fluid * steps / t_batched, 1), "unit": "particle-steps/s"}
 
 
@@ -178,6 +245,7 @@ def bench_adjoint(torch, vtp, scenes, nsteps=100, ckpt=10):
     t_both = time.perf_counter() - t0
     ctx.close()
     return {"workload": wl.name, "fluid_particles": nf, "steps": nsteps, "checkpoint_every": ckpt,
+            "max_neighbor_num": params.max_neighbor_num,
             "forward_ms": round(t_fwd * 1e3, 3), "forward_backward_ms": round(t_both * 1e3, 3),
             "cost_ratio": round(t_both / t_fwd, 3), "grad_xsph_k": gk}
 
@@ -201,8 +269,18 @@ def cpu_baseline(wl, params, budget_s=20.0):
         if el > budget_s or steps >= 50:
             break
     return {"value": wl.n_fluid * steps / el, "unit": "particle-steps/s", "cores": pyoracle.get_threads(),
-            "kind": "port",
+            "kind": "port", "cpu_model": cpu_model(), "host_logical_cpus": os.cpu_count(),
             "sample": f"FP64 oracle, {wl.name}: {steps} ticks after 1 untimed tick ({el:.1f} s)"}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 STAGE_KERNEL = {"pbf_lambda": "k_pbf_lambda", "pbf_dx": "k_pbf_dx", "pbf_xsph": "k_pbf_xsph",
@@ -319,9 +397,11 @@ def main():
     roof = None
     if kb is not None:
         ach = kb / (dom_ms * 1e-3) / 1e9
+        own = kernel_bytes_own_fields(dom, n, nf)
         roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
-                "kernel_ms": round(dom_ms, 5), "algorithmic_bytes": kb}
+                "kernel_ms": round(dom_ms, 5), "algorithmic_bytes": kb,
+                "own_field_bytes": own, "own_field_frac": round(own / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
         if TRAFFIC_JSON.exists():   # counter-measured HBM bytes per launch (rocprofv3 --pmc, calibrated)
             t = json.loads(TRAFFIC_JSON.read_text())["kernels"].get(dom)
             if t:
@@ -342,7 +422,7 @@ def main():
         if name in traffic:
             e["traffic_frac"] = round(traffic[name]["bytes"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)
         roof_all[name] = e
-    passes = 3 if cells * wl.scenes() >= (1 << 16) else 2
+    passes = key_passes(cells * wl.scenes())
     step_bytes = pbf_step_bytes(n, nf, cells * wl.scenes(), params.iteration_num, passes)
     st = ctx.stats()
     import numpy as np

@@ -145,7 +145,8 @@ typedef enum vtp_field {
     VTP_FIELD_VEL_PHASE = 19,   /* float[n][P][3]                                */
     VTP_FIELD_COLOR = 20,       /* float[n][3]                                   */
     VTP_FIELD_CD = 21,          /* float[n]   IMM-CT per-particle Cd             */
-    VTP_FIELD_Q = 22,           /* float[n][9] conformation tensor, row-major    */
+    VTP_FIELD_Q = 22,           /* float[n][9] conformation tensor, row-major
+                                   (held as Q - I on the device)                 */
     VTP_FIELD_VEL_GRAD = 23,    /* float[n][9]                                   */
     VTP_FIELD_VISE_STRESS = 24, /* float[n][9]                                   */
     VTP_FIELD_VEL_DRIFT = 25,   /* float[n][P][3]                                */
@@ -260,7 +261,8 @@ vtp_status vtp_grid_info(const vtp_ctx *ctx, uint32_t grid[3], uint32_t *cells);
  *   float  rest_density[n]
  *   float  color[n][4]             (w unused)
  *   IMM-CT only:
- *   float  Q[9][n]                 conformation tensor, row-major components
+ *   float  Q_minus_I[9][n]         conformation tensor minus the identity
+ *                                  (the device's representation), row-major
  * Loading into a context built with the same solver type and the same
  * attached objects (count, material, scene and size of each) continues the run
  * bit for bit: the ticks after vtp_load_state equal the ticks the saved

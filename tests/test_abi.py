@@ -174,7 +174,7 @@ def _image(solver, n, P):
 @pytest.mark.parametrize("solver,P,names", [
     (_abi.PBF, 0, ["orig", "x", "v"]),
     (_abi.IMM, 3, ["orig", "x", "v", "vol_frac", "vel_phase", "rest_density", "color"]),
-    (_abi.IMMCT, 2, ["orig", "x", "v", "vol_frac", "vel_phase", "rest_density", "color", "Q"]),
+    (_abi.IMMCT, 2, ["orig", "x", "v", "vol_frac", "vel_phase", "rest_density", "color", "Q_minus_I"]),
 ])
 def test_parse_state_sections(solver, P, names):
     data, parts = _image(solver, 37, P)

@@ -15,11 +15,23 @@ physics settings, untouched:
   surface tension 0.001.
 
 Checks: solver iteration counts equal the oracle's every tick; after tick 1
-the state and the solver fields (surface normal and both kappa arrays
-included) agree within the single-step bars used elsewhere (1e-5 L2-relative
-for x, v and the fields computed from them directly, 1e-4 for differences of
-near-equal sums); after tick 3 positions within 1e-5 and velocities within
-1e-4.
+the state and the solver fields (surface normal, alpha, compression ratio,
+the divergence solve's kappa) agree within the single-step bars used
+elsewhere (1e-5 L2-relative for x, v and the fields computed from them
+directly, 1e-4 for the normal); after tick 3 positions within 1e-5 and
+velocities within 1e-4.
+
+These scenes start at rest, uncompressed, and fall freely for ~50 ticks, so
+the density solve's predictor delta = max(cr - 1 + dt div v*, 0) is non-zero
+only where cr sits a few 1e-6 above 1: it is a difference of near-equal
+numbers, resolved in FP32 to about 64 eps |cr| (eps = 2^-24, cr ~ 1: 3.8e-6,
+the rounding of a 30-term sum of position-rounded kernel values).  delta is
+therefore compared with that absolute bound, kappa_incomp through
+kappa alpha dt^2 V (the same quantity, from the oracle's alpha, dt and V)
+with that bound plus 1e-5 relative, and the first-tick velocities and phase
+velocities of the fluid solvers, to which this pressure adds, within 5e-5
+(L2-relative).  The same fields are compared at 1e-4 on compressed scenes
+(test_gpu_dfsph.py, test_gpu_imm.py).
 """
 import json
 from pathlib import Path
@@ -37,15 +49,15 @@ REF = Path(__file__).resolve().parent / "golden" / "reference_configs" / "Exampl
 
 ONE_TICK = {
     "PBF": {"pos": 1e-5, "vel": 1e-5, "density": 1e-5, "lambda": 5e-5, "error": 5e-5},
-    "DFSPH": {"pos": 1e-5, "vel": 1e-5, "volume": 1e-5, "compression_ratio": 1e-5, "df_alpha": 1e-5,
-              "vel_adv": 1e-5, "surface_normal": 1e-4, "delta_cr": 1e-4, "kappa_div": 1e-4,
-              "kappa_incomp": 1e-4, "acc": 1e-4},
-    "IMM": {"pos": 1e-5, "vel": 1e-5, "mass": 1e-5, "volume": 1e-5, "rest_density": 1e-5, "vol_frac": 1e-5,
-            "vel_phase": 1e-5, "color": 1e-5, "surface_normal": 1e-4, "kappa_div": 1e-4, "kappa_incomp": 1e-4},
-    "IMMCT": {"pos": 1e-5, "vel": 1e-5, "mass": 1e-5, "volume": 1e-5, "rest_density": 1e-5, "vol_frac": 1e-5,
-              "vel_phase": 1e-5, "color": 1e-5, "surface_normal": 1e-4, "kappa_div": 1e-4,
-              "kappa_incomp": 1e-4, "Q": 1e-4, "vel_grad": 1e-4, "vise_stress": 1e-4, "cd": 1e-5, "vis": 1e-4},
+    "DFSPH": {"pos": 1e-5, "vel": 5e-5, "volume": 1e-5, "compression_ratio": 1e-5, "df_alpha": 1e-5,
+              "vel_adv": 5e-5, "surface_normal": 1e-4, "kappa_div": 1e-4},
+    "IMM": {"pos": 1e-5, "vel": 5e-5, "mass": 1e-5, "volume": 1e-5, "rest_density": 1e-5, "vol_frac": 1e-5,
+            "vel_phase": 5e-5, "color": 1e-5, "surface_normal": 1e-4, "kappa_div": 1e-4},
+    "IMMCT": {"pos": 1e-5, "vel": 5e-5, "mass": 1e-5, "volume": 1e-5, "rest_density": 1e-5, "vol_frac": 1e-5,
+              "vel_phase": 5e-5, "color": 1e-5, "surface_normal": 1e-4, "kappa_div": 1e-4,
+              "Q": 1e-4, "vel_grad": 1e-4, "vise_stress": 1e-4, "cd": 1e-5, "vis": 1e-4},
 }
+DELTA_ABS = 64 * 2.0 ** -24   # FP32 resolution of cr ~ 1 summed over ~30 neighbours
 
 
 def l2_rel(a, b):
@@ -61,6 +73,20 @@ def _compare(ctx, orc, fields):
     bad = {f: e for f, e in res.items() if not e <= fields[f]}
     assert not bad, f"over tolerance: {bad} (all {res})"
     return res
+
+
+def _compare_density_predictor(ctx, orc, dt, active):
+    ids = vtp._abi.FIELDS
+    d_g, d_r = ctx.read("delta_cr"), orc.read(ids["delta_cr"][0])
+    k_g, k_r = ctx.read("kappa_incomp"), orc.read(ids["kappa_incomp"][0])
+    alpha, vol = orc.read(ids["df_alpha"][0]), orc.read(ids["volume"][0])
+    fluid = ctx.read("mat") == vtp.MAT_FLUID
+    err_d = np.abs(d_g - d_r)[fluid].max()
+    assert err_d <= DELTA_ABS, f"delta: max abs error {err_d:.3e} > {DELTA_ABS:.3e}"
+    err_k = (np.abs(k_g - k_r) * alpha * dt * dt * vol - 1e-5 * np.abs(d_r))[fluid].max()
+    assert err_k <= DELTA_ABS, f"kappa_incomp: error {err_k:.3e} in delta units > {DELTA_ABS:.3e}"
+    if active:
+        assert (d_r[fluid] > 0).any()   # the predictor is active somewhere
 
 
 def _iters(ctx, orc):
@@ -88,6 +114,9 @@ def test_reference_example_scene(solver):
         assert g == r, f"tick {t + 1}: iterations gpu {g} oracle {r}"
         if t == 0:
             _compare(ctx, orc, ONE_TICK[solver])
+            if solver != "PBF":
+                # (the IMM-CT scene's first density solve has delta = 0 everywhere)
+                _compare_density_predictor(ctx, orc, float(np.float32(params.time_step)), solver != "IMMCT")
     _compare(ctx, orc, {"pos": 1e-5, "vel": 1e-4})
     assert ctx.stats()["crash_flags"] == 0
     if solver != "PBF":

@@ -435,7 +435,7 @@ static void alloc_tape(Ctx &c, TapeBufs &t) {
     t.perm.alloc(n);
     t.orig.alloc(n);
     t.cnt.alloc(n);
-    t.nbr.alloc((size_t)c.ugns.cap * c.ugns.stride + 1);
+    t.nbr.alloc(nbr_words(c.ugns.cap, c.ugns.stride));
     for (uint32_t m = 0; m <= K; ++m) t.x[m].alloc(n);
     for (uint32_t m = 0; m < K; ++m) {
         t.lam[m].alloc(n);

@@ -463,14 +463,17 @@ extern "C" vtp_status vtp_read(vtp_ctx *ctx, int32_t field, void *host_dst, size
                     for (size_t a = 0; a < comps; ++a)
                         out[((size_t)orig[i] * P + k) * comps + a] = t[(k * stride + i) * elem_floats + a];
         };
-        auto mat9 = [&](const float *d) {
+        // plus_identity: the device holds Q - I (full FP32 precision near equilibrium)
+        auto mat9 = [&](const float *d, bool plus_identity = false) {
             if (!d) throw std::runtime_error("field not available for this solver");
             const size_t stride = c.ugns.stride;
             std::vector<float> t(9 * stride);
             VTP_HIP(hipMemcpyAsync(t.data(), d, t.size() * 4, hipMemcpyDeviceToHost, c.stream));
             VTP_HIP(hipStreamSynchronize(c.stream));
             for (uint32_t i = 0; i < n; ++i)
-                for (size_t a = 0; a < 9; ++a) out[(size_t)orig[i] * 9 + a] = t[a * stride + i];
+                for (size_t a = 0; a < 9; ++a)
+                    out[(size_t)orig[i] * 9 + a] =
+                        t[a * stride + i] + ((plus_identity && (a == 0 || a == 4 || a == 8)) ? 1.0f : 0.0f);
         };
         switch (field) {
             case VTP_FIELD_POS: f4_xyz(c.X[0].p); break;
@@ -499,7 +502,8 @@ extern "C" vtp_status vtp_read(vtp_ctx *ctx, int32_t field, void *host_dst, size
             case VTP_FIELD_CD: f1(c.P ? c.cd.p : nullptr); break;
             case VTP_FIELD_Q: case VTP_FIELD_VEL_GRAD: case VTP_FIELD_VISE_STRESS:
                 if (c.p.solver != VTP_IMMCT) throw std::runtime_error("field only exists for IMM-CT");
-                mat9(field == VTP_FIELD_Q ? c.Q[0].p : field == VTP_FIELD_VEL_GRAD ? c.Lgrad.p : c.stress.p);
+                mat9(field == VTP_FIELD_Q ? c.Q[0].p : field == VTP_FIELD_VEL_GRAD ? c.Lgrad.p : c.stress.p,
+                     field == VTP_FIELD_Q);
                 break;
             default: throw std::runtime_error("field not available");
         }

@@ -73,7 +73,7 @@ __global__ void k_df_init(float4 *__restrict__ x, float4 *__restrict__ v, float 
         const uint32_t nn = w & ~kDeadBit;
         float d = (w & kDeadBit) ? 0.0f : q.k.w0;
         for (uint32_t t = 0; t < nn; ++t) {
-            uint32_t j = nbr[(size_t)t * stride + i];
+            uint32_t j = nbr_entry(nbr, stride, i, t);
             if (mat[j] != mi) continue;
             float4 xj = x[j];
             float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -489,7 +489,12 @@ void df_pressure_solve(Ctx &c, const float4 *x, const float4 *v, int which) {
     float *kap = which == 0 ? c.kappa_div.p : c.kappa.p;
     hipLaunchKernelGGL(k_solve_begin, dim3(1), dim3(256), 0, c.stream, c.status.p, c.scal.p, lp.nobj);
     uint32_t &hint = which == 0 ? c.iter_hint_div : c.iter_hint_inc;
-    uint32_t chunk = c.opt_chunk ? c.opt_chunk : std::max(1u, hint);
+    // The first chunk is the previous solve's iteration count; past it the
+    // host checks every max(1, hint/4) iterations, which bounds the no-op
+    // launches enqueued after the exit.  With stage timing every iteration
+    // is checked, so that only launches that did work are timed.
+    const uint32_t step = c.timer.enabled ? 1u : c.opt_chunk ? c.opt_chunk : std::max(1u, hint / 4);
+    uint32_t chunk = c.timer.enabled ? 1u : c.opt_chunk ? c.opt_chunk : std::max(1u, hint);
     uint32_t launched = 0;
     for (;;) {
         for (uint32_t it = 0; it < chunk && launched <= cap; ++it, ++launched) {
@@ -507,7 +512,7 @@ void df_pressure_solve(Ctx &c, const float4 *x, const float4 *v, int which) {
         }
         read_status(c);
         if (c.h_status[kStDone] || launched > cap) break;
-        chunk = std::min(chunk * 2, cap + 1);
+        chunk = step;
     }
     uint32_t iters = c.h_status[word];
     hint = iters;

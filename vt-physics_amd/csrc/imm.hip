@@ -137,7 +137,7 @@ __global__ void k_imm_init_a(float4 *__restrict__ x, float4 *__restrict__ v, flo
     if (q.ct) {
         vis[i] = q.vis_base;
 #pragma unroll
-        for (int a = 0; a < 9; ++a) Q[(size_t)a * ph.stride + i] = (a == 0 || a == 4 || a == 8) ? 1.0f : 0.0f;
+        for (int a = 0; a < 9; ++a) Q[(size_t)a * ph.stride + i] = 0.0f;   // Q - I: Q starts at I
     }
     (void)x;
     (void)mat;
@@ -160,7 +160,7 @@ __global__ void k_imm_boundary_rest(float4 *__restrict__ x, float4 *__restrict__
     }
     float r = rest_in[i];
     for (uint32_t t = 0; t < nn; ++t) {
-        uint32_t j = nbr[(size_t)t * stride + i];
+        uint32_t j = nbr_entry(nbr, stride, i, t);
         if (mat[j] == VTP_MAT_FLUID) r = fmaxf(r, rest_in[j]);
     }
     rest[i] = r;
@@ -170,7 +170,7 @@ __global__ void k_imm_boundary_rest(float4 *__restrict__ x, float4 *__restrict__
         const float4 xi = x[i];
         float d = (cnt[i] & kDeadBit) ? 0.0f : k.w0;
         for (uint32_t t = 0; t < nn; ++t) {
-            uint32_t j = nbr[(size_t)t * stride + i];
+            uint32_t j = nbr_entry(nbr, stride, i, t);
             if (mat[j] != mi) continue;
             float4 xj = x[j];
             float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -431,16 +431,20 @@ __global__ __launch_bounds__(256) void k_immct_conformation(const float4 *__rest
             L[3] = fmaf(uy, gx, L[3]); L[4] = fmaf(uy, gy, L[4]); L[5] = fmaf(uy, gz, L[5]);
             L[6] = fmaf(uz, gx, L[6]); L[7] = fmaf(uz, gy, L[7]); L[8] = fmaf(uz, gz, L[8]);
         });
+    // The state holds Q - I, not Q: near equilibrium Q - I is orders of
+    // magnitude below 1, and FP32 keeps its full relative precision only
+    // when it is stored as such (the stress mu (Q - I) would otherwise carry
+    // the rounding of 1 + (Q - I)).
     const size_t st = stride;
-    float Qm[9];
+    float QI[9], Qm[9];
 #pragma unroll
-    for (int a = 0; a < 9; ++a) Qm[a] = Q[a * st + i];
+    for (int a = 0; a < 9; ++a) {
+        QI[a] = Q[a * st + i];
+        Qm[a] = QI[a] + ((a == 0 || a == 4 || a == 8) ? 1.0f : 0.0f);
+    }
     const float ap = fpoly[i];
     const float beta = q.beta0 * fmaxf(0.0f, q.theta - ap) / (q.theta + 1e-6f);
     const float kap = powf(1.0f - ap, -1.2f);
-    float QI[9];
-#pragma unroll
-    for (int a = 0; a < 9; ++a) QI[a] = Qm[a] - ((a == 0 || a == 4 || a == 8) ? 1.0f : 0.0f);
     float Qn[9];
 #pragma unroll
     for (int r = 0; r < 3; ++r)
@@ -454,7 +458,7 @@ __global__ __launch_bounds__(256) void k_immct_conformation(const float4 *__rest
                 qiq += QI[3 * r + m] * Qm[3 * m + cc];    // ((Q - I) Q)_rc
             }
             float dq = ql + ltq - q.inv_relax * QI[3 * r + cc] - beta * qiq;
-            Qn[3 * r + cc] = Qm[3 * r + cc] + dq * dt;
+            Qn[3 * r + cc] = QI[3 * r + cc] + dq * dt;   // the new Q - I
         }
     const float tr = L[0] + L[4] + L[8];   // trace of D equals trace of L
     const float shear = sqrtf(0.5f * tr * tr);
@@ -466,7 +470,7 @@ __global__ __launch_bounds__(256) void k_immct_conformation(const float4 *__rest
     float S[9];
 #pragma unroll
     for (int a = 0; a < 9; ++a) {
-        S[a] = mu * (Qn[a] - ((a == 0 || a == 4 || a == 8) ? 1.0f : 0.0f));
+        S[a] = mu * Qn[a];
         Q[a * st + i] = Qn[a];
         Lout[a * st + i] = L[a];
         stress[a * st + i] = S[a];
@@ -572,12 +576,11 @@ __global__ __launch_bounds__(256) void k_imm_fluxes(const float4 *__restrict__ x
         if (fi0 != tag) {
             if (fi0) return;   // flagged in an earlier round: its zero fluxes stand
             const uint32_t nn = cnt[i] & ~kDeadBit;
-            const uint32_t *p = nbr + i;
             bool dirty = false;
             for (uint32_t t = 0; t < nn && !dirty; t += 8) {
                 uint32_t j[8];
 #pragma unroll
-                for (uint32_t b = 0; b < 8; ++b) j[b] = p[(size_t)min(t + b, nn - 1) * stride];
+                for (uint32_t b = 0; b < 8; ++b) j[b] = nbr_entry(nbr, stride, i, min(t + b, nn - 1));
 #pragma unroll
                 for (uint32_t b = 0; b < 8; ++b) dirty |= flag[j[b]] == tag;
             }

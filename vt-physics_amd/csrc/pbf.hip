@@ -219,7 +219,7 @@ __global__ void k_pbf_init(float4 *__restrict__ x, float4 *__restrict__ v, const
         float delta = (w & kDeadBit) ? 0.0f : k.w0;
     #pragma unroll 4
     for (uint32_t t = 0; t < nn; ++t) {
-            uint32_t j = nbr[(size_t)t * stride + i];
+            uint32_t j = nbr_entry(nbr, stride, i, t);
             if (mat[j] != mi) continue;
             float4 xj = x[j];
             float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;

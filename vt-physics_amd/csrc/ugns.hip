@@ -16,9 +16,10 @@
 //  * because the input is the previous step's slot order rather than attach
 //    order, equal cell ids are put in attach-id order by a per-cell rank pass,
 //    which reproduces thrust's stable (cellId, attachId) order exactly;
-//  * neighbour lists hold slot indices in an ELL layout nbr[t*stride + slot]
-//    (t-major: lane i of a wavefront reads word i, fully coalesced) without
-//    the self entry; the reference layout is rebuilt on request.
+//  * neighbour lists hold 16-bit slot offsets from a per-particle base in an
+//    ELL layout (column t of slot i at col[t*stride + i]: lane i of a
+//    wavefront reads halfword i, fully coalesced; layout in vtp_device.h)
+//    without the self entry; the reference layout is rebuilt on request.
 #include <algorithm>
 
 #include "vtp_internal.h"
@@ -30,16 +31,16 @@ constexpr int kRsRounds = 4;                    // keys per lane per tile
 constexpr int kRsWaveChunk = 64 * kRsRounds;    // 256 keys per wavefront
 constexpr int kRsTile = kRsThreads * kRsRounds; // 1024 keys per workgroup (1.3k tiles at 1.3M keys)
 // Neighbour build: 128 threads, each with a private LDS column of up to
-// kNbrStage words.
+// kNbrStage 16-bit entries.
 constexpr uint32_t kNbrBuildThreads = 128;
 #ifndef VTP_NBR_STAGE
-#define VTP_NBR_STAGE 16
+#define VTP_NBR_STAGE 32
 #endif
 #ifndef VTP_BUILD_BATCH
 #define VTP_BUILD_BATCH 8
 #endif
 #ifndef VTP_NBR_STAGE_LAMBDA
-#define VTP_NBR_STAGE_LAMBDA 24
+#define VTP_NBR_STAGE_LAMBDA 48
 #endif
 // At least 6 waves per SIMD: caps the build at 80 VGPRs.  The plain build
 // needs 59 (8 waves); the one that also sums lambda would take 81 (5 waves)
@@ -446,9 +447,9 @@ __device__ uint32_t reference_scan(const float4 *__restrict__ x, const uint32_t 
     const uint32_t gxy = (uint32_t)(g.gx * g.gy);
     uint32_t c = 0;
     // j == i gives d2 == 0, which the test rejects (the reference skips it)
-    auto test = [&](uint32_t j, const float4 &xj) {
+    auto test = [&](uint32_t j, const float4 &xj, uint32_t zl) {
         if (is_neighbor(dist2_exact(p, xyz(xj)), g.nbr_d2_max)) {
-            if (c < cap) put(c, j, xj);
+            if (c < cap) put(c, j, xj, zl);
             ++c;
         }
     };
@@ -474,7 +475,7 @@ __device__ uint32_t reference_scan(const float4 *__restrict__ x, const uint32_t 
                     for (uint32_t b = 0; b < B; ++b) a[b] = ldg(x, min(j + b, e));
 #pragma unroll
                     for (uint32_t b = 0; b < B; ++b)
-                        if (j + b <= e) test(j + b, a[b]);
+                        if (j + b <= e) test(j + b, a[b], (uint32_t)(oz + 1));
                 }
                 if (c >= cap) return cap;
             }
@@ -509,7 +510,16 @@ __device__ uint32_t reference_scan(const float4 *__restrict__ x, const uint32_t 
 // written (the walk holds every listed neighbour's position and mass), with
 // the lambda pass's own per-neighbour code and order, and stores the pass's
 // outputs, static particles' included.
-template <bool kStaged, int kLam>
+//
+// Entries are written as 16-bit (cell layer, offset) pairs from the
+// particle's per-layer base slots (layout in vtp_device.h).  A 64-slot group
+// in which any offset does not fit 14 bits is marked wide, and the
+// kWide instance of the kernel, launched after every build, walks the lists
+// of the marked groups again (same walk, same order) and stores them as full
+// 32-bit entries; every other group returns at once.  The kWide instance
+// neither counts caps nor computes lambda: the first instance already did,
+// over the same neighbours in the same order.
+template <bool kStaged, int kLam, bool kWide>
 __global__ __launch_bounds__(kNbrBuildThreads, VTP_BUILD_MIN_WAVES) void k_build_neighbors(const float4 *__restrict__ x,
                                                                       const uint32_t *__restrict__ key,
                                                                       const uint32_t *__restrict__ start,
@@ -521,17 +531,33 @@ __global__ __launch_bounds__(kNbrBuildThreads, VTP_BUILD_MIN_WAVES) void k_build
                                                                       uint32_t walk,
                                                                       const int32_t *__restrict__ fluid_only,
                                                                       PbfLambdaOut lo) {
-    extern __shared__ uint32_t s_list[];
+    extern __shared__ uint16_t s_list[];
+    // a thread's staged entries pair up in its own LDS words (entries 2k and
+    // 2k + 1 in word k of its column), so no two lanes write one word
+    auto stage_slot = [&](uint32_t c) { return ((c >> 1) * blockDim.x + threadIdx.x) * 2u + (c & 1u); };
     const uint32_t i = slot_index();
+    uint32_t *const wide_flag = nbr + 3 * (size_t)stride;
+    uint16_t *const col = nbr_cols(nbr, stride);
+    if (kWide && !__builtin_amdgcn_readfirstlane(i < stride ? wide_flag[i >> 6] : 0u)) return;
     // uniform over the grid; walk = 1 / 2 forces the row / reference walk
     const bool ref_walk = walk == 2 || (walk == 0 && status[kStNbrPrev] > n / kRefOrderShare);
-    const uint32_t stage = kStaged ? min(cap, kLam ? kNbrStageLambda : kNbrStage) : 0u;
+    const uint32_t stage = (kStaged && !kWide) ? min(cap, kLam ? kNbrStageLambda : kNbrStage) : 0u;
     float4 xi{};
     float rho = 0, gsx = 0, gsy = 0, gsz = 0, s2 = 0;
     bool sums = false;   // this particle's lambda sums are accumulated
-    auto put = [&](uint32_t c, uint32_t j, const float4 &xj) {
-        if (c < stage) s_list[c * blockDim.x + threadIdx.x] = j;
-        else nbr[(size_t)c * stride + i] = j;
+    uint32_t zb[3] = {kEmpty, kEmpty, kEmpty};   // lowest slot of each cell layer dz = -1, 0, +1
+    bool over = false;   // an offset from its layer's base does not fit 14 bits
+    auto put = [&](uint32_t c, uint32_t j, const float4 &xj, uint32_t zl) {
+        if (kWide) {
+            col[(size_t)(2 * c) * stride + i] = (uint16_t)(j & 0xFFFFu);
+            col[(size_t)(2 * c + 1) * stride + i] = (uint16_t)(j >> 16);
+            return;
+        }
+        const uint32_t o = j - (zl == 0 ? zb[0] : zl == 1 ? zb[1] : zb[2]);
+        over |= o > kNbrOffMask;
+        const uint16_t e = (uint16_t)((zl << kNbrOffBits) | (o & kNbrOffMask));
+        if (c < stage) s_list[stage_slot(c)] = e;
+        else col[(size_t)c * stride + i] = e;
         if (kLam && sums) pbf_lambda_pair<kLam == 2>(xi, xj, lo.q.k, rho, gsx, gsy, gsz, s2);
     };
     auto restart = [&]() {   // the self term first, as the lambda pass
@@ -565,14 +591,14 @@ __global__ __launch_bounds__(kNbrBuildThreads, VTP_BUILD_MIN_WAVES) void k_build
             const float3 p = kLam ? xyz(xi) : xyz(x[i]);
             sums = kLam && fluid;
             if (sums) restart();
-            auto test = [&](uint32_t j, const float4 &xj) {
+            auto test = [&](uint32_t j, const float4 &xj, uint32_t zl) {
                 // j == i gives d2 == 0, which the test rejects
                 if (is_neighbor(dist2_exact(p, xyz(xj)), g.nbr_d2_max)) {
-                    if (c < cap) put(c, j, xj);
+                    if (c < cap) put(c, j, xj, zl);
                     ++c;
                 }
             };
-            auto walk = [&](uint32_t j, uint32_t e) {
+            auto walk = [&](uint32_t j, uint32_t e, uint32_t zl) {
                 constexpr uint32_t B = kLam ? VTP_BUILD_BATCH_LAMBDA : VTP_BUILD_BATCH;   // candidates in flight
                 for (; j <= e; j += B) {
                     float4 a[B];
@@ -580,44 +606,62 @@ __global__ __launch_bounds__(kNbrBuildThreads, VTP_BUILD_MIN_WAVES) void k_build
                     for (uint32_t b = 0; b < B; ++b) a[b] = ldg(x, min(j + b, e));
 #pragma unroll
                     for (uint32_t b = 0; b < B; ++b)
-                        if (j + b <= e) test(j + b, a[b]);
+                        if (j + b <= e) test(j + b, a[b], zl);
+                }
+            };
+            const int X0 = max(cx - 1, 0), X1 = min(cx + 1, g.gx - 1);
+            // cell-table entries (start, end) of the 3 cells of row
+            // r = 3 (dz + 1) + (dy + 1), rows in key order; cells past X1
+            // repeat X1, and out-of-grid rows read as empty
+            uint32_t ts[3], te[3];
+            auto request = [&](int r) {
+                const int Z = cz + r / 3 - 1, Y = cy + r % 3 - 1;
+                const bool ok = Z >= 0 && Z < g.gz && Y >= 0 && Y < g.gy;
+                // clamped, so that the address is a cell of this scene even where !ok
+                const uint32_t row = sbase + (uint32_t)min(max(Z, 0), g.gz - 1) * gxy +
+                                     (uint32_t)min(max(Y, 0), g.gy - 1) * (uint32_t)g.gx;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const uint32_t cell = row + (uint32_t)min(X0 + q, X1);
+                    ts[q] = ok ? ldg(start, cell) : kEmpty;
+                    te[q] = ok ? ldg(end, cell) : kEmpty;
+                }
+            };
+            // the row's slot range [s, e] from the requested entries (s = kEmpty: empty row)
+            auto range = [&](uint32_t &s, uint32_t &e) {
+                s = kEmpty;
+                e = 0;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    if (ts[q] != kEmpty) {
+                        if (s == kEmpty) s = ts[q];
+                        e = te[q];
+                    }
                 }
             };
             if (ref_walk) {
+                // each layer's lowest slot: the start of its first non-empty
+                // row in key order
+                for (int r = 0; r < 9; ++r) {
+                    uint32_t s, e;
+                    request(r);
+                    range(s, e);
+                    if (zb[r / 3] == kEmpty) zb[r / 3] = s;
+                }
                 c = reference_scan(x, start, end, g, p, cx, cy, cz, sbase, cap, put);
             } else {
-                const int X0 = max(cx - 1, 0), X1 = min(cx + 1, g.gx - 1);
                 // The 9 row ranges in key order (z outer, y inner): a sorted
                 // list.  The cell-table entries of row r + 1 are requested
                 // before row r is walked, and a row's starts and ends are
-                // loaded together (cells past X1 repeat X1), so the table
-                // loads cost no round trip of their own.
-                uint32_t ts[3], te[3];
-                auto request = [&](int r) {
-                    const int Z = cz + r / 3 - 1, Y = cy + r % 3 - 1;
-                    const bool ok = Z >= 0 && Z < g.gz && Y >= 0 && Y < g.gy;
-                    // clamped, so that the address is a cell of this scene even where !ok
-                    const uint32_t row = sbase + (uint32_t)min(max(Z, 0), g.gz - 1) * gxy +
-                                         (uint32_t)min(max(Y, 0), g.gy - 1) * (uint32_t)g.gx;
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        const uint32_t cell = row + (uint32_t)min(X0 + q, X1);
-                        ts[q] = ok ? ldg(start, cell) : kEmpty;
-                        te[q] = ok ? ldg(end, cell) : kEmpty;
-                    }
-                };
+                // loaded together, so the table loads cost no round trip of
+                // their own.
                 request(0);
                 for (int r = 0; r < 9; ++r) {
-                    uint32_t s = kEmpty, e = 0;
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        if (ts[q] != kEmpty) {
-                            if (s == kEmpty) s = ts[q];
-                            e = te[q];
-                        }
-                    }
+                    uint32_t s, e;
+                    range(s, e);
                     if (r < 8) request(r + 1);
-                    if (s != kEmpty && c <= cap) walk(s, e);   // past the cap: rebuilt below
+                    if (zb[r / 3] == kEmpty) zb[r / 3] = s;   // before the layer's first entry
+                    if (s != kEmpty && c <= cap) walk(s, e, (uint32_t)(r / 3));   // past the cap: rebuilt below
                 }
                 if (c > cap) {
                     if (sums) restart();
@@ -629,18 +673,27 @@ __global__ __launch_bounds__(kNbrBuildThreads, VTP_BUILD_MIN_WAVES) void k_build
             if (sums) pbf_lambda_store(i, xi, rho, gsx, gsy, gsz, s2, lo.q, lo.lam, lo.dens, lo.errc, lo.sg, lo.xl);
         }
         if (kLam && !fluid) pbf_lambda_store_static(i, xi, lo.lam, lo.dens, lo.errc, lo.sg, lo.xl);
+        if (!kWide) {
+            nbr[i] = zb[0];   // kEmpty for a layer without particles: never referenced
+            nbr[stride + i] = zb[1];
+            nbr[2 * (size_t)stride + i] = zb[2];
+        }
     }
+    if (kWide) return;
     // particles that reached the cap steer the walk of the next build
     const uint64_t m = __ballot(full);
     if (m && (threadIdx.x & 63u) == 0) {
         atomicAdd(&status[kStNbrAtCap], (uint32_t)__popcll(m));
         atomicAdd(&status[kStCapSeen], (uint32_t)__popcll(m));
     }
-    if (kStaged) {
+    // one format per 64-slot group (a wave): wide if any offset overflowed
+    const bool wide = __ballot(over) != 0;
+    if ((threadIdx.x & 63u) == 0 && i < stride) wide_flag[i >> 6] = wide ? 1u : 0u;
+    if (kStaged && !wide) {
         // every lane reads back only its own LDS column: no barrier needed
         const uint32_t cmax = min(wave_max_u32(c), stage);
-        for (uint32_t col = 0; col < cmax; ++col)
-            if (col < c) nbr[(size_t)col * stride + i] = s_list[col * blockDim.x + threadIdx.x];
+        for (uint32_t cc = 0; cc < cmax; ++cc)
+            if (cc < c) col[(size_t)cc * stride + i] = s_list[stage_slot(cc)];
     }
 }
 
@@ -681,7 +734,7 @@ __global__ void k_export_neighbors(const uint32_t *__restrict__ nbr, const uint3
     };
     row[0] = p;
     for (uint32_t a = 0; a < c; ++a) {
-        uint32_t ja = nbr[(size_t)a * stride + i];
+        uint32_t ja = nbr_entry(nbr, stride, i, a);
         uint64_t ka = okey(ja);
         uint32_t b = a;
         // row[1..a] holds slot ids sorted so far (translated below)
@@ -729,7 +782,7 @@ void Ugns::allocate(uint32_t n_, const Grid &g_, uint32_t max_neighbors) {
     so.alloc(n);
     perm.alloc(n);
     orig_new.alloc(n);
-    nbr.alloc((size_t)cap * stride + 1);
+    nbr.alloc(nbr_words(cap, stride));
     cnt.alloc(n);
 }
 
@@ -786,15 +839,20 @@ void Ugns::build_neighbors(const float4 *x, hipStream_t s, StageTimer &tm, const
     if (!status) throw std::runtime_error("neighbour build without status words");
     static_rows = fluid_only == nullptr;
     tm.begin(s, "ugns_neighbors");
-    const size_t lds = (size_t)std::min(cap, lam ? kNbrStageLambda : kNbrStage) * kNbrBuildThreads * sizeof(uint32_t);
+    // staged entries pair up in 32-bit words: an odd stage takes a whole last word
+    const uint32_t staged = std::min(cap, lam ? kNbrStageLambda : kNbrStage);
+    const size_t lds = (size_t)((staged + 1) / 2) * 2 * kNbrBuildThreads * sizeof(uint16_t);
     const dim3 grid(blocks_for(n, kNbrBuildThreads)), block(kNbrBuildThreads);
 #define VTP_BUILD(LAM, OUT)                                                                                   \
-    hipLaunchKernelGGL((k_build_neighbors<true, LAM>), grid, block, lds, s, x, key[sorted].p, start.p, end.p, g, \
-                       n, stride, cap, nbr.p, cnt.p, status, walk, fluid_only, OUT)
+    hipLaunchKernelGGL((k_build_neighbors<true, LAM, false>), grid, block, lds, s, x, key[sorted].p, start.p,   \
+                       end.p, g, n, stride, cap, nbr.p, cnt.p, status, walk, fluid_only, OUT)
     if (!lam) VTP_BUILD(0, PbfLambdaOut{});
     else if (lam->q.mass_weighted) VTP_BUILD(2, *lam);
     else VTP_BUILD(1, *lam);
 #undef VTP_BUILD
+    // groups whose offsets overflowed 16 bits: the same lists as 32-bit entries
+    hipLaunchKernelGGL((k_build_neighbors<false, 0, true>), grid, block, 0, s, x, key[sorted].p, start.p, end.p, g,
+                       n, stride, cap, nbr.p, cnt.p, status, walk, fluid_only, PbfLambdaOut{});
     tm.end(s);
 }
 

@@ -198,21 +198,80 @@ __device__ __forceinline__ float ldg_w(const float4 *__restrict__ base, uint32_t
     return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + (uint64_t)(j * 16u + 12u));
 }
 
-// Walks the ELL list of slot i (column t at nbr + t*stride) in batches: the
-// gathers of a batch are issued together, and the indices of the next batch
-// are requested before the current batch is used, so the index -> gather
-// dependency of one batch overlaps the arithmetic of the previous one.  A
-// batch is 8 neighbours when a neighbour's data fits in 32 bytes and 4 when
-// it is larger (register pressure).  The column base is wave-uniform (t is
-// the same in every lane), so index loads are one 256-byte line per wave and
-// column.
-template <typename Load, typename Use>
-__device__ __forceinline__ void for_each_neighbor(const uint32_t *__restrict__ nbr, uint32_t stride, uint32_t i,
-                                                  uint32_t nn, Load &&load, Use &&use) {
-    const char *const base = reinterpret_cast<const char *>(nbr);
-    const size_t step = (size_t)stride * sizeof(uint32_t);
-    const uint32_t off = i * (uint32_t)sizeof(uint32_t);
-    auto idx = [&](uint32_t t) { return *reinterpret_cast<const uint32_t *>(base + t * step + off); };
+// ---- neighbour lists -------------------------------------------------------
+// One allocation per search (Ugns::nbr), addressed through its first word:
+//   uint32_t base[3][stride]       per cell layer dz = -1, 0, +1 of the
+//                                  particle's 27 cells: the lowest slot of the
+//                                  layer's 9 cells (the start of its first
+//                                  non-empty cell row in key order)
+//   uint32_t wide[flag words]      one word per 64-slot group: 1 = that group's
+//                                  lists hold full 32-bit entries
+//   uint16_t col[2 cap][stride]    column-major (ELL) entries: entry t of slot i
+//                                  is e = col[t][i] with base[e >> 14][i] +
+//                                  (e & 0x3FFF) in a narrow group, and
+//                                  col[2t][i] | col[2t+1][i] << 16 in a wide one
+// Keys are z-major, so the 9 cells of one layer span three cell rows and a
+// neighbour lies at most two rows (~2 gx cells) past its layer's base: 14
+// bits hold that unless a cell row carries more than ~5k particles, and then
+// only that particle's 64-slot group switches to 32-bit entries.  Half the
+// index bytes of a 32-bit list: a wave reads one 128-byte line per column.
+// stride is a multiple of 64; the flag words are padded so that the columns
+// start on a 256-byte boundary.
+constexpr uint32_t kNbrOffBits = 14;
+constexpr uint32_t kNbrOffMask = (1u << kNbrOffBits) - 1u;
+__host__ __device__ inline uint32_t nbr_flag_words(uint32_t stride) { return ((stride / 64u + 63u) / 64u) * 64u; }
+__host__ __device__ inline size_t nbr_words(uint32_t cap, uint32_t stride) {
+    return 3 * (size_t)stride + nbr_flag_words(stride) + (size_t)cap * stride;   // 2 cap u16 columns = cap words
+}
+__host__ __device__ inline uint16_t *nbr_cols(uint32_t *nbr, uint32_t stride) {
+    return reinterpret_cast<uint16_t *>(nbr + 3 * (size_t)stride + nbr_flag_words(stride));
+}
+__host__ __device__ inline const uint16_t *nbr_cols(const uint32_t *nbr, uint32_t stride) {
+    return reinterpret_cast<const uint16_t *>(nbr + 3 * (size_t)stride + nbr_flag_words(stride));
+}
+__host__ __device__ inline uint32_t nbr_flag_index(uint32_t stride, uint32_t i) { return 3 * stride + (i >> 6); }
+__device__ __forceinline__ uint32_t nbr_decode(uint32_t e, uint32_t b0, uint32_t b1, uint32_t b2) {
+    const uint32_t z = e >> kNbrOffBits;
+    return (z == 0 ? b0 : z == 1 ? b1 : b2) + (e & kNbrOffMask);
+}
+// Entry t of slot i's list, decoded per lane (for kernels off the hot path).
+__device__ __forceinline__ uint32_t nbr_entry(const uint32_t *__restrict__ nbr, uint32_t stride, uint32_t i,
+                                              uint32_t t) {
+    const uint16_t *col = nbr_cols(nbr, stride);
+    if (nbr[nbr_flag_index(stride, i)])
+        return (uint32_t)col[(size_t)(2 * t) * stride + i] | ((uint32_t)col[(size_t)(2 * t + 1) * stride + i] << 16);
+    return nbr_decode(col[(size_t)t * stride + i], nbr[i], nbr[stride + i], nbr[2 * (size_t)stride + i]);
+}
+
+// Column readers of one slot's list.  The column base is wave-uniform (t is
+// the same in every lane), and the address is a 32-bit byte offset from it.
+struct NarrowIdx {
+    const char *col;
+    size_t step;
+    uint32_t off, b0, b1, b2;
+    __device__ __forceinline__ uint32_t operator()(uint32_t t) const {
+        return nbr_decode(*reinterpret_cast<const uint16_t *>(col + t * step + off), b0, b1, b2);
+    }
+};
+struct WideIdx {
+    const char *col;
+    size_t step;
+    uint32_t off;
+    __device__ __forceinline__ uint32_t operator()(uint32_t t) const {
+        const uint32_t lo = *reinterpret_cast<const uint16_t *>(col + (2 * t) * step + off);
+        const uint32_t hi = *reinterpret_cast<const uint16_t *>(col + (2 * t + 1) * step + off);
+        return lo | (hi << 16);
+    }
+};
+
+// Walks slot i's list in batches: the gathers of a batch are issued
+// together, and the indices of the next batch are requested before the
+// current batch is used, so the index -> gather dependency of one batch
+// overlaps the arithmetic of the previous one.  A batch is 8 neighbours when
+// a neighbour's data fits in 32 bytes and 4 when it is larger (register
+// pressure).
+template <typename Idx, typename Load, typename Use>
+__device__ __forceinline__ void walk_list(const Idx &idx, uint32_t nn, Load &&load, Use &&use) {
     using D = decltype(load(0u));
     constexpr uint32_t kNbrBatch = sizeof(D) > 32 ? VTP_NBR_BATCH / 2 : VTP_NBR_BATCH;
     uint32_t t = 0;
@@ -247,23 +306,29 @@ __device__ __forceinline__ void for_each_neighbor(const uint32_t *__restrict__ n
     for (; t < nn; ++t) use(load(idx(t)));
 }
 
-// Cooperative walk: L lanes share one particle, lane `sub` taking columns
+// The list of slot i, nn entries.  The format flag belongs to i's 64-slot
+// group, which is the same in every lane of a wavefront here (waves cover
+// 64 consecutive, 64-aligned slots, or 64/L particles with L lanes each), so
+// it is read once per wave and the branch is uniform.
+template <typename Load, typename Use>
+__device__ __forceinline__ void for_each_neighbor(const uint32_t *__restrict__ nbr, uint32_t stride, uint32_t i,
+                                                  uint32_t nn, Load &&load, Use &&use) {
+    const char *col = reinterpret_cast<const char *>(nbr_cols(nbr, stride));
+    const size_t step = (size_t)stride * sizeof(uint16_t);
+    const uint32_t off = i * (uint32_t)sizeof(uint16_t);
+    if (__builtin_amdgcn_readfirstlane(nbr[nbr_flag_index(stride, i)]))
+        walk_list(WideIdx{col, step, off}, nn, load, use);
+    else
+        walk_list(NarrowIdx{col, step, off, nbr[i], nbr[stride + i], nbr[2 * (size_t)stride + i]}, nn, load, use);
+}
+
+// Cooperative walk: L lanes share one particle, lane `sub` taking entries
 // sub, sub + L, sub + 2L, ...  In one gather instruction the L lanes of a
 // particle then read consecutive entries of its list (mostly neighbouring
 // slots of one cell row), so a wave touches far fewer cache lines than with
 // one particle per lane; the partial sums are combined with lane_sum<L>.
-template <uint32_t L, typename Load, typename Use>
-__device__ __forceinline__ void for_each_neighbor_coop(const uint32_t *__restrict__ nbr, uint32_t stride,
-                                                       uint32_t i, uint32_t nn, uint32_t sub, Load &&load,
-                                                       Use &&use) {
-    if (L == 1) {
-        for_each_neighbor(nbr, stride, i, nn, load, use);
-        return;
-    }
-    const char *const base = reinterpret_cast<const char *>(nbr);
-    const size_t step = (size_t)stride * sizeof(uint32_t);
-    const uint32_t off = i * (uint32_t)sizeof(uint32_t);
-    auto idx = [&](uint32_t t) { return *reinterpret_cast<const uint32_t *>(base + t * step + off); };
+template <uint32_t L, typename Idx, typename Load, typename Use>
+__device__ __forceinline__ void walk_list_coop(const Idx &idx, uint32_t nn, uint32_t sub, Load &&load, Use &&use) {
     using D = decltype(load(0u));
     constexpr uint32_t B = sizeof(D) > 32 ? 2 : 4;   // neighbours in flight per lane
     uint32_t t = sub;
@@ -278,6 +343,24 @@ __device__ __forceinline__ void for_each_neighbor_coop(const uint32_t *__restric
         for (uint32_t b = 0; b < B; ++b) use(d[b]);
     }
     for (; t < nn; t += L) use(load(idx(t)));
+}
+
+template <uint32_t L, typename Load, typename Use>
+__device__ __forceinline__ void for_each_neighbor_coop(const uint32_t *__restrict__ nbr, uint32_t stride,
+                                                       uint32_t i, uint32_t nn, uint32_t sub, Load &&load,
+                                                       Use &&use) {
+    if (L == 1) {
+        for_each_neighbor(nbr, stride, i, nn, load, use);
+        return;
+    }
+    const char *col = reinterpret_cast<const char *>(nbr_cols(nbr, stride));
+    const size_t step = (size_t)stride * sizeof(uint16_t);
+    const uint32_t off = i * (uint32_t)sizeof(uint16_t);
+    if (__builtin_amdgcn_readfirstlane(nbr[nbr_flag_index(stride, i)]))
+        walk_list_coop<L>(WideIdx{col, step, off}, nn, sub, load, use);
+    else
+        walk_list_coop<L>(NarrowIdx{col, step, off, nbr[i], nbr[stride + i], nbr[2 * (size_t)stride + i]}, nn, sub,
+                          load, use);
 }
 
 // Sum over the L lanes (L = 1, 2 or 4, aligned groups) sharing one particle.

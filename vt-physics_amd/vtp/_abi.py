@@ -131,7 +131,7 @@ def state_sections(header: VtpStateHeader) -> list[tuple[str, np.dtype, tuple]]:
         out += [("vol_frac", np.float32, (P, n)), ("vel_phase", np.float32, (P, n, 4)),
                 ("rest_density", np.float32, (n,)), ("color", np.float32, (n, 4))]
     if header.solver == IMMCT:
-        out.append(("Q", np.float32, (9, n)))
+        out.append(("Q_minus_I", np.float32, (9, n)))
     return out
 
 
