@@ -443,13 +443,14 @@ __global__ void k_cell_fixup(const uint32_t *__restrict__ key, const uint32_t *_
 template <typename Put>
 __device__ uint32_t reference_scan(const float4 *__restrict__ x, const uint32_t *__restrict__ start,
                                    const uint32_t *__restrict__ end, const Grid &g, float3 p, int cx, int cy,
-                                   int cz, uint32_t sbase, uint32_t cap, Put &&put) {
+                                   int cz, uint32_t sbase, uint32_t cap, const uint32_t *__restrict__ bases,
+                                   uint32_t stride, uint32_t i, Put &&put) {
     const uint32_t gxy = (uint32_t)(g.gx * g.gy);
     uint32_t c = 0;
     // j == i gives d2 == 0, which the test rejects (the reference skips it)
-    auto test = [&](uint32_t j, const float4 &xj, uint32_t zl) {
+    auto test = [&](uint32_t j, const float4 &xj, uint32_t ztag) {
         if (is_neighbor(dist2_exact(p, xyz(xj)), g.nbr_d2_max)) {
-            if (c < cap) put(c, j, xj, zl);
+            if (c < cap) put(c, j, xj, ztag);
             ++c;
         }
     };
@@ -467,6 +468,8 @@ __device__ uint32_t reference_scan(const float4 *__restrict__ x, const uint32_t 
                 uint32_t j = ldg(start, cid);
                 if (j == kEmpty) continue;
                 const uint32_t e = ldg(end, cid);
+                // the entry tag of this cell's layer: its base minus (layer << 14)
+                const uint32_t ztag = bases[(size_t)(oz + 1) * stride + i] - ((uint32_t)(oz + 1) << kNbrOffBits);
                 // batches of B; the last one loads slot e again in place of
                 // slots past the cell and skips their tests
                 for (; j <= e && c < cap; j += B) {
@@ -475,7 +478,7 @@ __device__ uint32_t reference_scan(const float4 *__restrict__ x, const uint32_t 
                     for (uint32_t b = 0; b < B; ++b) a[b] = ldg(x, min(j + b, e));
 #pragma unroll
                     for (uint32_t b = 0; b < B; ++b)
-                        if (j + b <= e) test(j + b, a[b], (uint32_t)(oz + 1));
+                        if (j + b <= e) test(j + b, a[b], ztag);
                 }
                 if (c >= cap) return cap;
             }
@@ -545,17 +548,19 @@ __global__ __launch_bounds__(kNbrBuildThreads, VTP_BUILD_MIN_WAVES) void k_build
     float4 xi{};
     float rho = 0, gsx = 0, gsy = 0, gsz = 0, s2 = 0;
     bool sums = false;   // this particle's lambda sums are accumulated
-    uint32_t zb[3] = {kEmpty, kEmpty, kEmpty};   // lowest slot of each cell layer dz = -1, 0, +1
-    bool over = false;   // an offset from its layer's base does not fit 14 bits
-    auto put = [&](uint32_t c, uint32_t j, const float4 &xj, uint32_t zl) {
+    // A layer's offsets fit 14 bits when its rows span fewer than 2^14
+    // slots past its base; checked once per row, a group with any larger
+    // span is stored wide.
+    bool over = false;
+    // ztag = the entry's layer base - (layer << 14): the entry is j - ztag
+    // (its top two bits the layer) whenever the offset fits
+    auto put = [&](uint32_t c, uint32_t j, const float4 &xj, uint32_t ztag) {
         if (kWide) {
             col[(size_t)(2 * c) * stride + i] = (uint16_t)(j & 0xFFFFu);
             col[(size_t)(2 * c + 1) * stride + i] = (uint16_t)(j >> 16);
             return;
         }
-        const uint32_t o = j - (zl == 0 ? zb[0] : zl == 1 ? zb[1] : zb[2]);
-        over |= o > kNbrOffMask;
-        const uint16_t e = (uint16_t)((zl << kNbrOffBits) | (o & kNbrOffMask));
+        const uint16_t e = (uint16_t)(j - ztag);
         if (c < stage) s_list[stage_slot(c)] = e;
         else col[(size_t)c * stride + i] = e;
         if (kLam && sums) pbf_lambda_pair<kLam == 2>(xi, xj, lo.q.k, rho, gsx, gsy, gsz, s2);
@@ -591,14 +596,14 @@ __global__ __launch_bounds__(kNbrBuildThreads, VTP_BUILD_MIN_WAVES) void k_build
             const float3 p = kLam ? xyz(xi) : xyz(x[i]);
             sums = kLam && fluid;
             if (sums) restart();
-            auto test = [&](uint32_t j, const float4 &xj, uint32_t zl) {
+            auto test = [&](uint32_t j, const float4 &xj, uint32_t ztag) {
                 // j == i gives d2 == 0, which the test rejects
                 if (is_neighbor(dist2_exact(p, xyz(xj)), g.nbr_d2_max)) {
-                    if (c < cap) put(c, j, xj, zl);
+                    if (c < cap) put(c, j, xj, ztag);
                     ++c;
                 }
             };
-            auto walk = [&](uint32_t j, uint32_t e, uint32_t zl) {
+            auto walk = [&](uint32_t j, uint32_t e, uint32_t ztag) {
                 constexpr uint32_t B = kLam ? VTP_BUILD_BATCH_LAMBDA : VTP_BUILD_BATCH;   // candidates in flight
                 for (; j <= e; j += B) {
                     float4 a[B];
@@ -606,7 +611,7 @@ __global__ __launch_bounds__(kNbrBuildThreads, VTP_BUILD_MIN_WAVES) void k_build
                     for (uint32_t b = 0; b < B; ++b) a[b] = ldg(x, min(j + b, e));
 #pragma unroll
                     for (uint32_t b = 0; b < B; ++b)
-                        if (j + b <= e) test(j + b, a[b], zl);
+                        if (j + b <= e) test(j + b, a[b], ztag);
                 }
             };
             const int X0 = max(cx - 1, 0), X1 = min(cx + 1, g.gx - 1);
@@ -639,16 +644,24 @@ __global__ __launch_bounds__(kNbrBuildThreads, VTP_BUILD_MIN_WAVES) void k_build
                     }
                 }
             };
+            // Each cell layer's base (its lowest slot, the start of its first
+            // non-empty row in key order; kEmpty for an empty layer) goes to
+            // the list buffer as soon as the layer's rows are known: only the
+            // current layer's base stays in a register, and the reference
+            // traversal reads them back.
             if (ref_walk) {
-                // each layer's lowest slot: the start of its first non-empty
-                // row in key order
+                uint32_t zbase = kEmpty;
+#pragma unroll 1
                 for (int r = 0; r < 9; ++r) {
                     uint32_t s, e;
                     request(r);
                     range(s, e);
-                    if (zb[r / 3] == kEmpty) zb[r / 3] = s;
+                    if (r % 3 == 0) zbase = kEmpty;
+                    if (zbase == kEmpty) zbase = s;
+                    over |= s != kEmpty && e - zbase > kNbrOffMask;
+                    if (r % 3 == 2) nbr[(size_t)(r / 3) * stride + i] = zbase;
                 }
-                c = reference_scan(x, start, end, g, p, cx, cy, cz, sbase, cap, put);
+                c = reference_scan(x, start, end, g, p, cx, cy, cz, sbase, cap, nbr, stride, i, put);
             } else {
                 // The 9 row ranges in key order (z outer, y inner): a sorted
                 // list.  The cell-table entries of row r + 1 are requested
@@ -656,16 +669,22 @@ __global__ __launch_bounds__(kNbrBuildThreads, VTP_BUILD_MIN_WAVES) void k_build
                 // loaded together, so the table loads cost no round trip of
                 // their own.
                 request(0);
+                uint32_t zbase = kEmpty;
+#pragma unroll 1
                 for (int r = 0; r < 9; ++r) {
                     uint32_t s, e;
                     range(s, e);
                     if (r < 8) request(r + 1);
-                    if (zb[r / 3] == kEmpty) zb[r / 3] = s;   // before the layer's first entry
-                    if (s != kEmpty && c <= cap) walk(s, e, (uint32_t)(r / 3));   // past the cap: rebuilt below
+                    if (r % 3 == 0) zbase = kEmpty;
+                    if (zbase == kEmpty) zbase = s;   // before the layer's first entry
+                    over |= s != kEmpty && e - zbase > kNbrOffMask;
+                    if (s != kEmpty && c <= cap)   // past the cap: rebuilt below
+                        walk(s, e, zbase - ((uint32_t)(r / 3) << kNbrOffBits));
+                    if (r % 3 == 2) nbr[(size_t)(r / 3) * stride + i] = zbase;
                 }
                 if (c > cap) {
                     if (sums) restart();
-                    c = reference_scan(x, start, end, g, p, cx, cy, cz, sbase, cap, put);
+                    c = reference_scan(x, start, end, g, p, cx, cy, cz, sbase, cap, nbr, stride, i, put);
                 }
             }
             cnt[i] = c;
@@ -673,11 +692,6 @@ __global__ __launch_bounds__(kNbrBuildThreads, VTP_BUILD_MIN_WAVES) void k_build
             if (sums) pbf_lambda_store(i, xi, rho, gsx, gsy, gsz, s2, lo.q, lo.lam, lo.dens, lo.errc, lo.sg, lo.xl);
         }
         if (kLam && !fluid) pbf_lambda_store_static(i, xi, lo.lam, lo.dens, lo.errc, lo.sg, lo.xl);
-        if (!kWide) {
-            nbr[i] = zb[0];   // kEmpty for a layer without particles: never referenced
-            nbr[stride + i] = zb[1];
-            nbr[2 * (size_t)stride + i] = zb[2];
-        }
     }
     if (kWide) return;
     // particles that reached the cap steer the walk of the next build
