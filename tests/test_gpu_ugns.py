@@ -279,3 +279,76 @@ def test_ugns_per_tick_search_bit_exact(solver, walk, radix):
     assert min(moved) > wl.n // 10, moved
     assert deads == ([2, 2, 2, 2] if solver == "PBF" else [2, 3, 5, 6]), deads
     ctx.close()
+
+
+def _layer_spans(cs, ce, grid, keys):
+    """Largest slot span of one cell layer's three rows (x-1..x+1) over the
+    particles with the given cell keys: the offsets a 16-bit list entry holds."""
+    gx, gy, gz = grid
+    worst = 0
+    for k in np.unique(keys):
+        cz, rem = divmod(int(k), gx * gy)
+        cy, cx = divmod(rem, gx)
+        for z in (cz - 1, cz, cz + 1):
+            if not 0 <= z < gz:
+                continue
+            lo, hi = None, None
+            for y in (cy - 1, cy, cy + 1):
+                if not 0 <= y < gy:
+                    continue
+                for x in (cx - 1, cx, cx + 1):
+                    if not 0 <= x < gx:
+                        continue
+                    c = (z * gy + y) * gx + x
+                    if cs[c] != 0xFFFFFFFF:
+                        lo = cs[c] if lo is None else lo
+                        hi = ce[c]
+            if lo is not None:
+                worst = max(worst, int(hi) - int(lo))
+    return worst
+
+
+@pytest.mark.parametrize("walk", [1, 2])
+def test_ugns_wide_lists_bit_exact(walk):
+    """A sheet a few cell rows high and 4,000 cells long: the rows of one
+    cell layer lie ~24k slots apart, more than a 14-bit list offset holds, so
+    those 64-slot groups store 32-bit entries (the fallback build).  The
+    lists (reference layout) equal the oracle's, and three PBF ticks through
+    the wide lists match the oracle's."""
+    s = 0.02
+    nx, ny, nz = 8000, 4, 2
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    pos = np.stack([0.03 + s * ix, 0.03 + s * iy, 0.03 + s * iz], -1).reshape(-1, 3)
+    pos = scenes.jitter(pos.astype(np.float32), s, seed=12)
+    cfg = scenes.pbf_config(lb=(0, 0, 0), size=(nx * s + 0.2, 0.2, 0.16))
+    cfg["PBF"]["Optional"] = {"enable": True, "gravity": [0.0, 0.0, 0.0]}
+    wl = scenes.Workload("sheet", "PBF", cfg, [scenes.Obj(vtp.MAT_FLUID, pos)])
+    params = vtp.params_from_json("PBF", wl.config_json())
+    ctx = vtp.Context(params)
+    ctx.set_option("neighbor_walk", walk)
+    ctx.set_option("static_lists", 1)
+    ctx.add_particles(pos, vtp.MAT_FLUID)
+    ctx.initialize()
+    grid, cells = ctx.grid_info()
+    orc = pyoracle.Oracle.from_workload(params, wl)
+    orc.initialize()
+    r = orc.neighbors(cells)
+    # the span that decides the format, from the oracle's cell tables
+    gx, gy, gz = grid
+    cell_of = lambda p: ((np.floor(p[:, 2] / (4 * params.particle_radius)).astype(np.int64) * gy +
+                          np.floor(p[:, 1] / (4 * params.particle_radius)).astype(np.int64)) * gx +
+                         np.floor(p[:, 0] / (4 * params.particle_radius)).astype(np.int64))
+    keys = cell_of(pos.astype(np.float64))
+    assert _layer_spans(r["cell_start"], r["cell_end"], grid, keys[::7]) > (1 << 14)
+    print(f"wide lists walk={walk}: {pos.shape[0]} particles", flush=True)
+    g = ctx.debug_neighbors(1)
+    np.testing.assert_array_equal(g["neighbor_num"], r["neighbor_num"])
+    np.testing.assert_array_equal(g["neighbors"], r["neighbors"])
+    for _ in range(3):
+        ctx.step(1)
+        orc.step(1)
+    g, r = ctx.debug_neighbors(1), orc.neighbors(cells)
+    np.testing.assert_array_equal(g["neighbors"], r["neighbors"])
+    x, xr = ctx.read("pos"), orc.read(0)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) <= 1e-5
+    ctx.close()
