@@ -290,34 +290,86 @@ TRAFFIC_JSON = ROOT / "profiles" / "pbf_dam_break_1m_traffic.json"
 
 def traffic_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, out=TRAFFIC_JSON):
     """HBM bytes per launch of the neighbour-pass kernels from two rocprofv3
-    --pmc runs (FETCH_SIZE, WRITE_SIZE, KiB per dispatch), each scaled by what
-    the same counter reads for a plain device copy of known size."""
+    --pmc runs (FETCH_SIZE, WRITE_SIZE, KiB per dispatch; the median over a
+    kernel's launches), each scaled by what the same counter reads for a
+    plain device copy of known size."""
     import csv
     from collections import defaultdict
 
-    def per_kernel(path, counter):
+    def dispatches(path, counter):
         acc = defaultdict(list)
         for r in csv.DictReader(open(path)):
             if r["Counter_Name"] == counter:
                 acc[r["Kernel_Name"]].append(float(r["Counter_Value"]))
-        return {k: sum(v) / len(v) for k, v in acc.items()}
+        return acc
+
+    def per_kernel(path, counter):
+        return {k: sum(v) / len(v) for k, v in dispatches(path, counter).items()}
+
+    def per_tick(path, counter, kname):
+        """KiB per launch, summed over the kernel's instantiations launched
+        every tick (those launched at least half as often as the most
+        launched one: the build and its 32-bit fallback pass, not the
+        search at initialisation)."""
+        acc = {k: v for k, v in dispatches(path, counter).items() if kname in k}
+        if not acc:
+            return None
+        top = max(len(v) for v in acc.values())
+        return sum(sorted(v)[len(v) // 2] for v in acc.values() if 2 * len(v) >= top)
 
     cf = max(per_kernel(copy_fetch_csv, "FETCH_SIZE").values())
     cw = max(per_kernel(copy_write_csv, "WRITE_SIZE").values())
     copy_kib = (1 << 30) / 1024   # calibrate_copy: 1 GiB read and 1 GiB written per dispatch
     fetch_scale, write_scale = copy_kib / cf, copy_kib / cw
-    fetch, write = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
     kernels = {}
     for stage, kname in STAGE_KERNEL.items():
-        fk = [v for k, v in fetch.items() if kname in k]
-        wk = [v for k, v in write.items() if kname in k]
-        if fk and wk:
-            rb, wb = max(fk) * 1024 * fetch_scale, max(wk) * 1024 * write_scale
+        fk, wk = per_tick(fetch_csv, "FETCH_SIZE", kname), per_tick(write_csv, "WRITE_SIZE", kname)
+        if fk is not None and wk is not None:
+            rb, wb = fk * 1024 * fetch_scale, wk * 1024 * write_scale
             kernels[stage] = {"read_bytes": round(rb), "write_bytes": round(wb), "bytes": round(rb + wb)}
     doc = {"workload": "pbf_dam_break_1m", "fetch_scale": round(fetch_scale, 4), "write_scale": round(write_scale, 4),
            "kernels": kernels}
     Path(out).write_text(json.dumps(doc, indent=1) + "\n")
     print(json.dumps(doc))
+
+
+LEG_FIRST_KERNEL = (("dfsph_1m", "k_df_"), ("batched_64", "k_pbf_"), ("adjoint", "k_adj_"), ("immct_1m", "k_imm"))
+
+
+def trace_summary(trace_csv):
+    """Markdown tables from a rocprofv3 --kernel-trace CSV of a default bench
+    run: per kernel and leg, launches and microseconds per launch.  A leg
+    starts at the first dispatch of its own kernel family, in the order the
+    bench runs them (PBF 1M headline, DFSPH 1M, batched, adjoint, IMM-CT 1M)."""
+    import csv
+    from collections import defaultdict
+
+    rows = sorted(((int(r["Start_Timestamp"]), int(r["End_Timestamp"]),
+                    r["Kernel_Name"].split("(")[0].replace("void ", "").replace("vtp::", ""),
+                    int(r["Grid_Size_X"]))
+                   for r in csv.DictReader(open(trace_csv))))
+    legs, leg, want = [], "pbf_1m (headline)", list(LEG_FIRST_KERNEL)
+    for t0, t1, name, grid in rows:
+        if want and name.startswith(want[0][1]):
+            leg = want.pop(0)[0]
+        legs.append((leg, name, grid, t1 - t0))
+    acc = defaultdict(list)
+    for leg, name, grid, d in legs:
+        acc[(leg, name, grid)].append(d)
+    order = [k[0] for k in LEG_FIRST_KERNEL]
+    for leg in ["pbf_1m (headline)"] + order:
+        items = [(k, v) for k, v in acc.items() if k[0] == leg]
+        if not items:
+            continue
+        tot = sum(sum(v) for _, v in items)
+        print(f"\n### {leg}: {tot / 1e6:.3f} ms of kernel time\n")
+        print("| kernel | grid | launches | avg us | min us | max us | total ms | % of leg |")
+        print("|---|---:|---:|---:|---:|---:|---:|---:|")
+        for (leg_, name, grid), v in sorted(items, key=lambda kv: -sum(kv[1]))[:16]:
+            if sum(v) < 0.005 * tot:
+                continue
+            print(f"| `{name}` | {grid} | {len(v)} | {sum(v) / len(v) / 1e3:.2f} | {min(v) / 1e3:.2f} | "
+                  f"{max(v) / 1e3:.2f} | {sum(v) / 1e6:.3f} | {100 * sum(v) / tot:.2f} |")
 
 
 def calibrate_copy(torch, nbytes=1 << 30, reps=5):
@@ -338,8 +390,8 @@ def calibrate_copy(torch, nbytes=1 << 30, reps=5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--only-headline", action="store_true", help="skip the DFSPH / batched / adjoint / IMM-CT legs")
@@ -347,9 +399,14 @@ def main():
                     help="only run 1 GiB device copies (counter calibration for the traffic figure)")
     ap.add_argument("--traffic-summary", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
                     help="turn rocprofv3 --pmc counter files into " + str(TRAFFIC_JSON.relative_to(ROOT)))
+    ap.add_argument("--trace-summary", metavar="KERNEL_TRACE_CSV",
+                    help="print per-leg kernel tables from a rocprofv3 --kernel-trace CSV of a bench run")
     args = ap.parse_args()
     if args.traffic_summary:
         traffic_summary(*args.traffic_summary)
+        return
+    if args.trace_summary:
+        trace_summary(args.trace_summary)
         return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1 "
