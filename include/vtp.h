@@ -233,6 +233,13 @@ uint32_t vtp_particle_count(const vtp_ctx *ctx);
 vtp_status vtp_read(vtp_ctx *ctx, int32_t field, void *host_dst, size_t bytes);
 /* Overwrites VTP_FIELD_POS or VTP_FIELD_VEL (attach order) after initialize. */
 vtp_status vtp_write(vtp_ctx *ctx, int32_t field, const void *host_src, size_t bytes);
+/* vtp_read / vtp_write of VTP_FIELD_POS or VTP_FIELD_VEL on DEVICE memory of
+ * the context's GPU (float[n][3], attach order), with no host copy: the same
+ * cudaMemcpy of exportData / the state upload, for callers whose arrays are
+ * already on the GPU (the torch autograd wrapper).  The caller's writes to
+ * d_src must be complete; both return after the context's stream. */
+vtp_status vtp_read_device(vtp_ctx *ctx, int32_t field, float *d_dst, size_t bytes);
+vtp_status vtp_write_device(vtp_ctx *ctx, int32_t field, const float *d_src, size_t bytes);
 vtp_status vtp_stats(vtp_ctx *ctx, vtp_step_stats *out);
 
 /* UGNS state of the last neighbour search, in the reference's own layout
@@ -301,14 +308,18 @@ vtp_status vtp_load_state(vtp_ctx *ctx, const void *host_src, size_t bytes);
 
 /* ---- execution options ---------------------------------------------------
  * Keys: "graph" (1 = capture the PBF substep in a hipGraph and replay it),
- * "autotune" (1 = time launch shapes of the PBF/DFSPH neighbour passes, at
- * initialize or immediately when already initialized, and keep the fastest;
- * the state is restored afterwards),
+ * "autotune" (time launch shapes of the neighbour passes of the context's
+ * solver, at initialize or immediately when already initialized, and keep
+ * the fastest; each candidate runs from a state image that is loaded back
+ * afterwards, so the simulation is not advanced.  1 = the workgroup size of
+ * each pass (64, 128 or 256; results bit-identical); 2 = also 1, 2 or 4
+ * lanes per particle in the PBF passes, which changes the FP32 rounding of
+ * neighbour sums.  The chosen shapes read back through vtp_get_option),
  * "block" (threads per block for the neighbour passes: 64, 128, 192 or 256;
  * 0 = default 256),
  * "stage_timing" (1 = record hipEvents around every stage; see
  * vtp_stage_times), "solver_chunk" (iterations launched per host check),
- * "lanes" (1 or 4 lanes per particle in the PBF neighbour passes; default 1),
+ * "lanes" (1, 2 or 4 lanes per particle in the PBF neighbour passes; default 1),
  * "neighbor_walk" (0 = adaptive, default: the list build walks cell rows and
  * switches to the reference's cell order once many lists reach the cap;
  * 1 = always rows; 2 = always the reference's order.  The neighbour sets are
@@ -328,6 +339,12 @@ vtp_status vtp_load_state(vtp_ctx *ctx, const void *host_src, size_t bytes);
  * computes the first constraint iteration's density and lambda from the
  * neighbours it lists; 0 = a separate pass does.  Bit-identical). */
 vtp_status vtp_set_option(vtp_ctx *ctx, const char *key, double value);
+/* Reads an option back: the keys above except "stage_timing" and
+ * "solver_chunk", and "block.<pass>" for the workgroup size a neighbour pass
+ * launches with (pbf_lambda, pbf_dx, pbf_xsph, df_precompute, df_delta_kappa,
+ * df_update_vadv, df_vis_normal, imm_precompute, imm_vis_normal,
+ * imm_tension_phase_vel, imm_fluxes, immct_conformation, immct_stress_acc). */
+vtp_status vtp_get_option(vtp_ctx *ctx, const char *key, double *value);
 
 /* Per-stage device time (ms, summed over launches) and launch counts,
  * recorded with hipEvents on the context's stream while "stage_timing" is 1.
@@ -346,10 +363,18 @@ vtp_status vtp_reset_stage_times(vtp_ctx *ctx);
 vtp_status vtp_pbf_rollout_fwd(vtp_ctx *ctx, uint32_t nsteps, uint32_t ckpt_every);
 vtp_status vtp_pbf_rollout_bwd(vtp_ctx *ctx, const float *grad_x, const float *grad_v,
                                float *grad_v0, float *grad_x0, float *grad_xsph_k);
+/* vtp_pbf_rollout_bwd with every array (and dL/dXSPH_k, one float) in DEVICE
+ * memory of the context's GPU; no host round trip.  Returns after the
+ * context's stream. */
+vtp_status vtp_pbf_rollout_bwd_device(vtp_ctx *ctx, const float *d_grad_x, const float *d_grad_v,
+                                      float *d_grad_v0, float *d_grad_x0, float *d_grad_xsph_k);
 /* Mean squared density deviation of the fluid at the current positions over
  * the last neighbour lists, L = mean_f (rho_f/rho0 - 1)^2 (unclamped SPH
  * density), and dL/dx (float[n][3], attach order; may be NULL). */
 vtp_status vtp_pbf_density_loss(vtp_ctx *ctx, double *loss, float *grad_x);
+/* vtp_pbf_density_loss with the loss (one double) and dL/dx (may be NULL) in
+ * DEVICE memory of the context's GPU; its scratch is allocated per call. */
+vtp_status vtp_pbf_density_loss_device(vtp_ctx *ctx, double *d_loss, float *d_grad_x);
 /* vtp_pbf_density_loss without any allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate (code under hipGraph
  * capture of their own, tight loops, harnesses that forbid hipMalloc).

@@ -7,6 +7,16 @@ import numpy as np
 import pytest
 
 ROOT = Path(__file__).resolve().parent.parent
+
+# torch (when importable) brings up its HIP runtime before any test module
+# loads libvtp.so, which then binds to that same runtime; the other order
+# leaves torch reporting no GPU, and the torch-tensor tests need both.
+try:
+    import torch
+    torch.cuda.is_available()
+except Exception:
+    pass
+
 sys.path.insert(0, str(ROOT / "vt-physics_amd"))
 sys.path.insert(0, str(ROOT / "oracle"))
 

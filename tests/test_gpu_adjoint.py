@@ -129,6 +129,37 @@ def test_torch_autograd_function():
     assert abs(float(k.grad) - ref_k) <= 1e-4 * abs(ref_k)
 
 
+def test_torch_autograd_on_device_tensors_matches_host_path():
+    """The same rollout and loss with tensors on the GPU (device-pointer entry
+    points, no host copies) and on the CPU (host entry points): the rollout
+    and the density loss bit for bit (the same kernels on the same state);
+    the gradients within 1e-5, since torch's own reductions of the loss
+    terms round differently on the two devices."""
+    torch = pytest.importorskip("torch")
+    from vtp.autodiff import DifferentiablePBF
+    wl = scene(seed=7)
+    p, ctx = setup(wl)
+    nf = wl.n_fluid
+    dpbf = DifferentiablePBF(ctx, STEPS, CKPT)
+    v_init = ctx.read("vel")
+    target = torch.tensor([0.7, 0.2, 0.5], dtype=torch.float64)
+    out = {}
+    for dev in ("cpu", "cuda"):
+        v0 = torch.tensor(v_init, device=dev, requires_grad=True)
+        k = torch.tensor(float(p.xsph_k), device=dev, requires_grad=True)
+        xt, vt = dpbf.rollout(v0, k)
+        assert xt.device.type == dev and vt.device.type == dev
+        dl = dpbf.density_loss(xt)
+        loss = ((xt[:nf].double().mean(0) - target.to(dev)) ** 2).sum() + dl + (vt[:nf].double() ** 2).mean()
+        loss.backward()
+        out[dev] = [t.detach().cpu().numpy() for t in (xt, vt, dl, v0.grad, k.grad)]
+    for name, a, b in zip(("x_T", "v_T", "density loss"), out["cpu"], out["cuda"]):
+        np.testing.assert_array_equal(a, b, err_msg=name)
+    np.testing.assert_allclose(out["cuda"][3], out["cpu"][3], rtol=1e-5, atol=1e-9 * np.abs(out["cpu"][3]).max())
+    assert abs(float(out["cuda"][4]) - float(out["cpu"][4])) <= 1e-5 * abs(float(out["cpu"][4]))
+    assert np.abs(out["cuda"][3]).max() > 0 and out["cuda"][4] != 0
+
+
 def _cluster_ctx(k, K=1, xsph=0.0, seed=2):
     """k^3 jittered fluid lattice at spacing 0.018 (no walls) with N(0, 0.1^2)
     velocities, config-0 physics otherwise."""

@@ -1,0 +1,105 @@
+"""Launch-shape autotuner (option "autotune") for every solver.
+
+Mode 1 tunes the workgroup size of each neighbour pass.  A particle's sums
+do not depend on it, so a tuned context must continue bit for bit like an
+untuned one; tuning runs its probe ticks from a state image and loads it
+back, so the step count, simulated time and iteration totals are unchanged
+by it.  Mode 2 (PBF) also tries 1, 2 and 4 lanes per particle, which changes
+the order of the neighbour sums: there the check is the one-step parity bar
+against the FP64 oracle, for each lane count.
+"""
+import numpy as np
+import pytest
+
+import pyoracle
+import vtp
+from vtp import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(wl, **opts):
+    p = vtp.params_from_json(wl.solver, wl.config_json())
+    ctx = vtp.Context(p)
+    for o in wl.objects:
+        ctx.add_particles(o.pos, o.mat, o.vel, o.vol_frac, o.scene)
+    for k, v in opts.items():
+        ctx.set_option(k, v)
+    ctx.initialize()
+    return p, ctx
+
+
+WORKLOADS = {
+    "PBF": lambda: scenes.cfg0_pbf(n_side=(16, 16, 16)),
+    "DFSPH": lambda: scenes.cfg1_dfsph(n_side=(20, 20, 20), cfl=True),
+    "IMM": lambda: scenes.cfg2_immct(n_side=(12, 12, 12), box=(0.3, 0.3, 0.3), solver="IMM"),
+    "IMMCT": lambda: scenes.cfg2_immct(n_side=(12, 12, 12), box=(0.3, 0.3, 0.3), solver="IMMCT"),
+}
+STATE = {"PBF": ("pos", "vel"), "DFSPH": ("pos", "vel"), "IMM": ("pos", "vel", "vol_frac", "vel_phase"),
+         "IMMCT": ("pos", "vel", "vol_frac", "vel_phase", "Q")}
+COUNTERS = ("steps", "sim_time", "total_div_iterations", "total_incomp_iterations", "crash_flags")
+
+
+@pytest.mark.parametrize("solver", sorted(WORKLOADS))
+def test_autotune_keeps_results_bit_identical(solver):
+    wl = WORKLOADS[solver]()
+    print(f"autotune {solver}: {wl.n} particles", flush=True)
+    _, a = _ctx(wl)
+    _, b = _ctx(wl)
+    a.step(3)
+    b.step(3)
+    before = {k: b.stats()[k] for k in COUNTERS}
+    b.set_option("autotune", 1)
+    assert {k: b.stats()[k] for k in COUNTERS} == before   # the probe ticks left no trace
+    for f in STATE[solver]:
+        np.testing.assert_array_equal(a.read(f), b.read(f), err_msg=f"{f} after tuning")
+    shapes = b.launch_shapes(vtp._abi.SOLVER_IDS[solver])
+    assert all(v in (64, 128, 256) for k, v in shapes.items() if k != "lanes") and shapes["lanes"] == 1
+    print(f"tuned shapes: {shapes}", flush=True)
+    a.step(4)
+    b.step(4)
+    for f in STATE[solver]:
+        np.testing.assert_array_equal(a.read(f), b.read(f), err_msg=f"{f} after 4 more ticks")
+    sa, sb = a.stats(), b.stats()
+    assert {k: sa[k] for k in COUNTERS} == {k: sb[k] for k in COUNTERS}
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 4])
+def test_pbf_lanes_one_step_parity(lanes):
+    """One PBF tick with each lane count against the FP64 oracle (1e-5 for
+    x, v and density; 5e-5 for C and lambda, the cancellation bound of the
+    PBF tests), graph and eager equal bit for bit."""
+    wl = scenes.cfg0_pbf(n_side=(16, 16, 16))
+    p, g = _ctx(wl, lanes=lanes, graph=1)
+    _, e = _ctx(wl, lanes=lanes, graph=0)
+    orc = pyoracle.Oracle.from_workload(p, wl)
+    orc.initialize()
+    g.step(1)
+    e.step(1)
+    orc.step(1)
+    np.testing.assert_array_equal(g.read("pos"), e.read("pos"))
+    ids = vtp._abi.FIELDS
+    for f, tol in (("pos", 1e-5), ("vel", 1e-5), ("density", 1e-5), ("error", 5e-5), ("lambda", 5e-5)):
+        a = g.read(f).astype(np.float64).ravel()
+        r = orc.read(ids[f][0]).astype(np.float64).ravel()
+        err = np.linalg.norm(a - r) / np.linalg.norm(r)
+        assert err <= tol, (lanes, f, err)
+    g.close()
+    e.close()
+
+
+def test_pbf_autotune_mode_2_picks_a_lane_count():
+    wl = scenes.cfg0_pbf(n_side=(16, 16, 16))
+    _, c = _ctx(wl)
+    c.step(2)
+    x = c.read("pos")
+    c.set_option("autotune", 2)
+    np.testing.assert_array_equal(c.read("pos"), x)
+    shapes = c.launch_shapes(vtp._abi.PBF)
+    assert shapes["lanes"] in (1, 2, 4)
+    print(f"mode-2 shapes at {wl.n} particles: {shapes}", flush=True)
+    c.step(3)
+    assert c.stats()["crash_flags"] == 0
+    c.close()

@@ -48,6 +48,34 @@ __global__ void k_adj_clear_word(uint32_t *w) {
     if (threadIdx.x == 0) *w = 0u;
 }
 
+// Terminal adjoints of the device entry point: attach-order float[n][3]
+// gradients (either may be null) into the final state's slot order, zero for
+// non-fluid particles, as the host entry point fills them.
+__global__ void k_adj_terminal(const float *__restrict__ gx, const float *__restrict__ gv,
+                               const uint32_t *__restrict__ orig, const int32_t *__restrict__ mat_attach, uint32_t n,
+                               float4 *__restrict__ xb, float4 *__restrict__ vb) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t a = orig[i];
+    const bool fluid = mat_attach[a] == VTP_MAT_FLUID;
+    const size_t o = 3 * (size_t)a;
+    xb[i] = (gx && fluid) ? make_float4(gx[o], gx[o + 1], gx[o + 2], 0.0f) : make_float4(0, 0, 0, 0);
+    vb[i] = (gv && fluid) ? make_float4(gv[o], gv[o + 1], gv[o + 2], 0.0f) : make_float4(0, 0, 0, 0);
+}
+
+// dL/dv0, dL/dx0 (slot order of the rollout's first state) to attach order,
+// and dL/dXSPH_k to one float, on the device.
+__global__ void k_adj_outputs(const uint32_t *__restrict__ orig, const float4 *__restrict__ xb,
+                              const float4 *__restrict__ vb, uint32_t n, float *__restrict__ gv0,
+                              float *__restrict__ gx0, const double *__restrict__ kbar, float *__restrict__ gk) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gk && i == 0) *gk = (float)*kbar;
+    if (i >= n) return;
+    const size_t o = 3 * (size_t)orig[i];
+    if (gv0) { const float4 v = vb[i]; gv0[o] = v.x; gv0[o + 1] = v.y; gv0[o + 2] = v.z; }
+    if (gx0) { const float4 x = xb[i]; gx0[o] = x.x; gx0[o + 1] = x.y; gx0[o + 2] = x.z; }
+}
+
 __global__ void k_adj_mat_from_orig(const uint32_t *__restrict__ orig, const int32_t *__restrict__ mat_attach,
                                     const uint32_t *__restrict__ scene_attach, uint32_t n, int32_t *__restrict__ mat,
                                     uint32_t *__restrict__ scene) {
@@ -621,8 +649,10 @@ extern "C" vtp_status vtp_pbf_rollout_fwd(vtp_ctx *ctx, uint32_t nsteps, uint32_
     });
 }
 
-extern "C" vtp_status vtp_pbf_rollout_bwd(vtp_ctx *ctx, const float *grad_x, const float *grad_v, float *grad_v0,
-                                          float *grad_x0, float *grad_k) {
+// The backward of both entry points; `device`: the gradients in and out are
+// device pointers on the context's GPU, else host arrays.
+static vtp_status rollout_bwd(vtp_ctx *ctx, const float *grad_x, const float *grad_v, float *grad_v0,
+                              float *grad_x0, float *grad_k, bool device) {
     if (!ctx) return VTP_EINVAL;
     Ctx &c = ctx->c;
     vtp_status s = adj_check(c);
@@ -650,21 +680,28 @@ extern "C" vtp_status vtp_pbf_rollout_bwd(vtp_ctx *ctx, const float *grad_x, con
         A.xb2.p = A.canon[2];
         A.vb2.p = A.canon[3];
         // terminal adjoints: attach order -> final slot order (non-fluid zero)
-        std::vector<uint32_t> orig(n);
-        std::vector<int32_t> mat(n);
-        VTP_HIP(hipMemcpyAsync(orig.data(), A.ckpt[A.nseg]->orig.p, n * 4, hipMemcpyDeviceToHost, c.stream));
-        VTP_HIP(hipStreamSynchronize(c.stream));
-        std::vector<float4> hx(n), hv(n);
-        for (uint32_t i = 0; i < n; ++i) {
-            uint32_t a = orig[i];
-            bool fluid = c.h_mat[a] == VTP_MAT_FLUID;
-            hx[i] = (grad_x && fluid) ? make_float4(grad_x[3 * a], grad_x[3 * a + 1], grad_x[3 * a + 2], 0)
-                                      : make_float4(0, 0, 0, 0);
-            hv[i] = (grad_v && fluid) ? make_float4(grad_v[3 * a], grad_v[3 * a + 1], grad_v[3 * a + 2], 0)
-                                      : make_float4(0, 0, 0, 0);
+        std::vector<uint32_t> orig;
+        std::vector<float4> hx, hv;
+        if (device) {
+            hipLaunchKernelGGL(k_adj_terminal, dim3(div_up(n, 256)), dim3(256), 0, c.stream, grad_x, grad_v,
+                               A.ckpt[A.nseg]->orig.p, c.mat_attach.p, n, A.xb.p, A.vb.p);
+        } else {
+            orig.resize(n);
+            hx.resize(n);
+            hv.resize(n);
+            VTP_HIP(hipMemcpyAsync(orig.data(), A.ckpt[A.nseg]->orig.p, n * 4, hipMemcpyDeviceToHost, c.stream));
+            VTP_HIP(hipStreamSynchronize(c.stream));
+            for (uint32_t i = 0; i < n; ++i) {
+                uint32_t a = orig[i];
+                bool fluid = c.h_mat[a] == VTP_MAT_FLUID;
+                hx[i] = (grad_x && fluid) ? make_float4(grad_x[3 * a], grad_x[3 * a + 1], grad_x[3 * a + 2], 0)
+                                          : make_float4(0, 0, 0, 0);
+                hv[i] = (grad_v && fluid) ? make_float4(grad_v[3 * a], grad_v[3 * a + 1], grad_v[3 * a + 2], 0)
+                                          : make_float4(0, 0, 0, 0);
+            }
+            VTP_HIP(hipMemcpyAsync(A.xb.p, hx.data(), n * sizeof(float4), hipMemcpyHostToDevice, c.stream));
+            VTP_HIP(hipMemcpyAsync(A.vb.p, hv.data(), n * sizeof(float4), hipMemcpyHostToDevice, c.stream));
         }
-        VTP_HIP(hipMemcpyAsync(A.xb.p, hx.data(), n * sizeof(float4), hipMemcpyHostToDevice, c.stream));
-        VTP_HIP(hipMemcpyAsync(A.vb.p, hv.data(), n * sizeof(float4), hipMemcpyHostToDevice, c.stream));
         VTP_HIP(hipMemsetAsync(A.kbar.p, 0, sizeof(double), c.stream));
         const bool graphs = !c.timer.enabled;   // stage timing needs eager launches
         for (int seg = (int)A.nseg - 1; seg >= 0; --seg) {
@@ -707,6 +744,18 @@ extern "C" vtp_status vtp_pbf_rollout_bwd(vtp_ctx *ctx, const float *grad_x, con
         }
         VTP_HIP(hipGetLastError());
         // adjoints are now in the slot order of the rollout's first state
+        if (device) {
+            hipLaunchKernelGGL(k_adj_outputs, dim3(div_up(n, 256)), dim3(256), 0, c.stream, A.ckpt[0]->orig.p,
+                               A.xb.p, A.vb.p, n, grad_v0, grad_x0, A.kbar.p, grad_k);
+            VTP_HIP(hipGetLastError());
+            load_ckpt(c, *A.ckpt[A.nseg]);   // leave the context at the rollout's final state
+            VTP_HIP(hipStreamSynchronize(c.stream));
+            c.lists_stale = true;
+            return VTP_OK;
+        }
+        orig.resize(n);
+        hx.resize(n);
+        hv.resize(n);
         VTP_HIP(hipMemcpyAsync(orig.data(), A.ckpt[0]->orig.p, n * 4, hipMemcpyDeviceToHost, c.stream));
         VTP_HIP(hipMemcpyAsync(hx.data(), A.xb.p, n * sizeof(float4), hipMemcpyDeviceToHost, c.stream));
         VTP_HIP(hipMemcpyAsync(hv.data(), A.vb.p, n * sizeof(float4), hipMemcpyDeviceToHost, c.stream));
@@ -724,6 +773,16 @@ extern "C" vtp_status vtp_pbf_rollout_bwd(vtp_ctx *ctx, const float *grad_x, con
         c.lists_stale = true;
         return VTP_OK;
     });
+}
+
+extern "C" vtp_status vtp_pbf_rollout_bwd(vtp_ctx *ctx, const float *grad_x, const float *grad_v, float *grad_v0,
+                                          float *grad_x0, float *grad_k) {
+    return rollout_bwd(ctx, grad_x, grad_v, grad_v0, grad_x0, grad_k, false);
+}
+
+extern "C" vtp_status vtp_pbf_rollout_bwd_device(vtp_ctx *ctx, const float *d_grad_x, const float *d_grad_v,
+                                                 float *d_grad_v0, float *d_grad_x0, float *d_grad_k) {
+    return rollout_bwd(ctx, d_grad_x, d_grad_v, d_grad_v0, d_grad_x0, d_grad_k, true);
 }
 
 extern "C" vtp_status vtp_pbf_density_loss(vtp_ctx *ctx, double *loss, float *grad_x) {
@@ -838,6 +897,36 @@ extern "C" vtp_status vtp_pbf_density_loss_noalloc(vtp_ctx *ctx, double *d_loss,
                                u.cnt.p, n, u.stride, q, nullptr, d_grad_x, c.orig.p);
         VTP_HIP(hipGetLastError());
         VTP_HIP(hipStreamSynchronize(c.stream));
+        return VTP_OK;
+    });
+}
+
+extern "C" vtp_status vtp_pbf_density_loss_device(vtp_ctx *ctx, double *d_loss, float *d_grad_x) {
+    if (!ctx) return VTP_EINVAL;
+    Ctx &c = ctx->c;
+    if (c.p.solver != VTP_PBF) { c.err = "density loss exists for PBF only"; return VTP_EINVAL; }
+    if (!c.initialized) { c.err = "not initialized"; return VTP_ESTATE; }
+    if (c.lists_stale) { c.err = "neighbour lists are stale after a rollout backward; tick first"; return VTP_ESTATE; }
+    if (!d_loss) { c.err = "vtp_pbf_density_loss_device: null loss pointer"; return VTP_EINVAL; }
+    return adj_guarded(c, [&]() -> vtp_status {
+        const uint32_t n = c.n, nb = div_up(n, 256);
+        const PbfK q = pbf_constants(c);
+        uint32_t nf = 0;
+        for (int32_t m : c.h_mat) nf += m == VTP_MAT_FLUID;
+        DevBuf<float> rb;
+        DevBuf<double> acc;
+        rb.alloc(n);
+        acc.alloc(1);
+        Ugns &u = c.ugns;
+        hipLaunchKernelGGL(k_dl_clear, dim3(1), dim3(64), 0, c.stream, acc.p);
+        hipLaunchKernelGGL(k_dl_rho, dim3(nb), dim3(256), 0, c.stream, c.X[0].p, c.mat.p, u.nbr.p, u.cnt.p, n,
+                           u.stride, q, nf ? 1.0f / nf : 0.0f, rb.p, acc.p);
+        hipLaunchKernelGGL(k_dl_finish, dim3(1), dim3(64), 0, c.stream, acc.p, (double)nf, d_loss);
+        if (d_grad_x)
+            hipLaunchKernelGGL(k_dl_grad, dim3(nb), dim3(256), 0, c.stream, c.X[0].p, rb.p, c.mat.p, u.nbr.p,
+                               u.cnt.p, n, u.stride, q, nullptr, d_grad_x, c.orig.p);
+        VTP_HIP(hipGetLastError());
+        VTP_HIP(hipStreamSynchronize(c.stream));   // before the scratch is freed
         return VTP_OK;
     });
 }

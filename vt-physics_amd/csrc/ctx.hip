@@ -329,7 +329,7 @@ extern "C" vtp_status vtp_initialize(vtp_ctx *ctx) {
         }
         read_status(c);
         c.initialized = true;
-        if (c.opt_autotune) autotune(c);   // requested before initialize
+        if (c.opt_autotune) autotune(c, c.opt_autotune);   // requested before initialize
         return VTP_OK;
     });
 }
@@ -540,6 +540,65 @@ extern "C" vtp_status vtp_write(vtp_ctx *ctx, int32_t field, const void *host_sr
     });
 }
 
+namespace vtp {
+// xyz of a slot-order float4 field to attach-order float[n][3], and back
+// (keeping w; `dst2`, if given, receives the same xyz: v* follows v).
+__global__ void k_xyz_to_attach(const float4 *__restrict__ src, const uint32_t *__restrict__ orig, uint32_t n,
+                                float *__restrict__ dst) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 v = src[i];
+    const size_t o = 3 * (size_t)orig[i];
+    dst[o] = v.x;
+    dst[o + 1] = v.y;
+    dst[o + 2] = v.z;
+}
+__global__ void k_xyz_from_attach(const float *__restrict__ src, const uint32_t *__restrict__ orig, uint32_t n,
+                                  float4 *__restrict__ dst, float4 *__restrict__ dst2) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t o = 3 * (size_t)orig[i];
+    float4 v = dst[i];
+    v.x = src[o];
+    v.y = src[o + 1];
+    v.z = src[o + 2];
+    dst[i] = v;
+    if (dst2) dst2[i] = v;
+}
+}  // namespace vtp
+
+extern "C" vtp_status vtp_read_device(vtp_ctx *ctx, int32_t field, float *d_dst, size_t bytes) {
+    CTX_GUARD(ctx);
+    if (!c.initialized) return ctx_fail(c, VTP_ESTATE, "vtp_read_device: not initialized");
+    if (field != VTP_FIELD_POS && field != VTP_FIELD_VEL)
+        return ctx_fail(c, VTP_EINVAL, "vtp_read_device: only POS and VEL");
+    if (!d_dst || bytes != (size_t)c.n * 12) return ctx_fail(c, VTP_EINVAL, "vtp_read_device: size mismatch");
+    return guarded(c, [&]() -> vtp_status {
+        hipLaunchKernelGGL(k_xyz_to_attach, dim3(div_up(c.n, 256)), dim3(256), 0, c.stream,
+                           field == VTP_FIELD_POS ? c.X[0].p : c.V[0].p, c.orig.p, c.n, d_dst);
+        VTP_HIP(hipGetLastError());
+        VTP_HIP(hipStreamSynchronize(c.stream));
+        return VTP_OK;
+    });
+}
+
+extern "C" vtp_status vtp_write_device(vtp_ctx *ctx, int32_t field, const float *d_src, size_t bytes) {
+    CTX_GUARD(ctx);
+    if (!c.initialized) return ctx_fail(c, VTP_ESTATE, "vtp_write_device: not initialized");
+    if (field != VTP_FIELD_POS && field != VTP_FIELD_VEL)
+        return ctx_fail(c, VTP_EINVAL, "vtp_write_device: only POS and VEL are writable");
+    if (!d_src || bytes != (size_t)c.n * 12) return ctx_fail(c, VTP_EINVAL, "vtp_write_device: size mismatch");
+    return guarded(c, [&]() -> vtp_status {
+        float4 *dst = field == VTP_FIELD_POS ? c.X[0].p : c.V[0].p;
+        float4 *dst2 = field == VTP_FIELD_VEL ? c.vadv.p : nullptr;
+        hipLaunchKernelGGL(k_xyz_from_attach, dim3(div_up(c.n, 256)), dim3(256), 0, c.stream, d_src, c.orig.p, c.n,
+                           dst, dst2);
+        VTP_HIP(hipGetLastError());
+        VTP_HIP(hipStreamSynchronize(c.stream));
+        return VTP_OK;
+    });
+}
+
 extern "C" vtp_status vtp_stats(vtp_ctx *ctx, vtp_step_stats *out) {
     CTX_GUARD(ctx);
     if (!out) return ctx_fail(c, VTP_EINVAL, "vtp_stats: null output");
@@ -624,8 +683,10 @@ extern "C" vtp_status vtp_set_option(vtp_ctx *ctx, const char *key, double value
             c.opt_graph = value != 0.0;
             pbf_capture_reset(c);
         } else if (k == "autotune") {
-            c.opt_autotune = value != 0.0;
-            if (c.opt_autotune && c.initialized) autotune(c);
+            if (value != 0.0 && value != 1.0 && value != 2.0)
+                throw std::runtime_error("autotune must be 0, 1 (workgroup sizes) or 2 (also PBF lanes)");
+            c.opt_autotune = (int)value;
+            if (c.opt_autotune && c.initialized) autotune(c, c.opt_autotune);
             pbf_capture_reset(c);
         } else if (k == "block") {
             uint32_t b = (uint32_t)value;
@@ -638,7 +699,7 @@ extern "C" vtp_status vtp_set_option(vtp_ctx *ctx, const char *key, double value
         } else if (k == "solver_chunk") {
             c.opt_chunk = (uint32_t)value;
         } else if (k == "lanes") {
-            if (value != 1.0 && value != 4.0) throw std::runtime_error("lanes must be 1 or 4");
+            if (value != 1.0 && value != 2.0 && value != 4.0) throw std::runtime_error("lanes must be 1, 2 or 4");
             c.opt_lanes = (uint32_t)value;
             pbf_capture_reset(c);
         } else if (k == "static_lists") {
@@ -662,6 +723,23 @@ extern "C" vtp_status vtp_set_option(vtp_ctx *ctx, const char *key, double value
         }
         return VTP_OK;
     });
+}
+
+extern "C" vtp_status vtp_get_option(vtp_ctx *ctx, const char *key, double *value) {
+    CTX_GUARD(ctx);
+    if (!key || !value) return ctx_fail(c, VTP_EINVAL, "vtp_get_option: null argument");
+    const std::string k(key);
+    if (k == "graph") *value = c.opt_graph;
+    else if (k == "autotune") *value = c.opt_autotune;
+    else if (k == "block") *value = c.opt_block;
+    else if (k == "lanes") *value = c.opt_lanes;
+    else if (k == "static_lists") *value = c.opt_static_lists;
+    else if (k == "build_lambda") *value = c.opt_build_lambda;
+    else if (k == "radix_sort") *value = c.ugns.radix;
+    else if (k == "neighbor_walk") *value = c.ugns.walk;
+    else if (k.rfind("block.", 0) == 0) *value = c.block(k.c_str() + 6);
+    else return ctx_fail(c, VTP_EINVAL, "vtp_get_option: unknown option '" + k + "'");
+    return VTP_OK;
 }
 
 extern "C" int32_t vtp_stage_times(vtp_ctx *ctx, char *names, size_t names_len, double *ms, uint64_t *launches,

@@ -491,11 +491,12 @@ void df_pressure_solve(Ctx &c, const float4 *x, const float4 *v, int which) {
     uint32_t &hint = which == 0 ? c.iter_hint_div : c.iter_hint_inc;
     // The first chunk is the previous solve's iteration count; past it the
     // host checks every max(1, hint/4) iterations, which bounds the no-op
-    // launches enqueued after the exit.  With stage timing every iteration
-    // is checked, so that only launches that did work are timed.
-    const uint32_t step = c.timer.enabled ? 1u : c.opt_chunk ? c.opt_chunk : std::max(1u, hint / 4);
-    uint32_t chunk = c.timer.enabled ? 1u : c.opt_chunk ? c.opt_chunk : std::max(1u, hint);
+    // launches enqueued after the exit.  The stage timer drops the events of
+    // those no-op iterations.
+    const uint32_t step = c.opt_chunk ? c.opt_chunk : std::max(1u, hint / 4);
+    uint32_t chunk = c.opt_chunk ? c.opt_chunk : std::max(1u, hint);
     uint32_t launched = 0;
+    const size_t tmark = c.timer.mark();
     for (;;) {
         for (uint32_t it = 0; it < chunk && launched <= cap; ++it, ++launched) {
             c.timer.begin(c.stream, name_dk);
@@ -515,6 +516,7 @@ void df_pressure_solve(Ctx &c, const float4 *x, const float4 *v, int which) {
         chunk = step;
     }
     uint32_t iters = c.h_status[word];
+    if (c.timer.enabled) c.timer.drop_from(tmark + 2 * (size_t)iters);   // two timed launches per iteration
     hint = iters;
     if (which == 0) c.total_div += iters; else c.total_inc += iters;
 }

@@ -876,10 +876,16 @@ static void imm_step_impl(Ctx &c) {
     u.build_neighbors(x, c.stream, c.timer);
     if (q.ct)
         hipLaunchKernelGGL(k_extract_phase, dim3(nb), dim3(256), 0, c.stream, ph.frac, c.fpoly.p, n, st, q.pp);
-    const uint32_t B = c.block("imm_pass");
-    const uint32_t nbB = div_up(n, B);
+    // workgroup size per neighbour pass (the tuner's choice, else "imm_pass")
+    auto shape = [&](const char *pass) {
+        auto it = c.block_of.find(pass);
+        const uint32_t b = it != c.block_of.end() ? it->second : c.block("imm_pass");
+        return std::make_pair(dim3(div_up(n, b)), dim3(b));
+    };
+    const auto s_pre = shape("imm_precompute"), s_vis = shape("imm_vis_normal"),
+               s_ten = shape("imm_tension_phase_vel"), s_flux = shape("imm_fluxes");
     c.timer.begin(c.stream, "imm_precompute");
-    hipLaunchKernelGGL(k_imm_precompute, dim3(nbB), dim3(B), 0, c.stream, x, v, c.vadv.p, c.mat.p, u.nbr.p, u.cnt.p, n,
+    hipLaunchKernelGGL(k_imm_precompute, s_pre.first, s_pre.second, 0, c.stream, x, v, c.vadv.p, c.mat.p, u.nbr.p, u.cnt.p, n,
                        st, q, c.fpoly.p, c.cr.p, c.alpha.p, c.cd.p, c.g1.p);
     c.timer.end(c.stream);
     df_pressure_solve(c, x, v, 0);
@@ -888,20 +894,21 @@ static void imm_step_impl(Ctx &c) {
                        c.mat.p, rest, c.cd.p, n, q, ph, c.scal.p, 0);
     c.timer.end(c.stream);
     c.timer.begin(c.stream, "imm_vis_normal");
-    hipLaunchKernelGGL(k_imm_vis_normal<MP>, dim3(nbB), dim3(B), 0, c.stream, x, v, c.mat.p, rest, c.cd.p, u.nbr.p,
+    hipLaunchKernelGGL(k_imm_vis_normal<MP>, s_vis.first, s_vis.second, 0, c.stream, x, v, c.mat.p, rest, c.cd.p, u.nbr.p,
                        u.cnt.p, n, st, q, ph, c.normal.p);
     c.timer.end(c.stream);
     c.timer.begin(c.stream, "imm_tension_phase_vel");
-    hipLaunchKernelGGL(k_imm_tension_phase_vel<MP>, dim3(nbB), dim3(B), 0, c.stream, x, v, c.vadv.p, c.mat.p, rest,
+    hipLaunchKernelGGL(k_imm_tension_phase_vel<MP>, s_ten.first, s_ten.second, 0, c.stream, x, v, c.vadv.p, c.mat.p, rest,
                        c.normal.p, u.nbr.p, u.cnt.p, n, st, q, ph, c.scal.p);
     c.timer.end(c.stream);
     if (q.ct) {
         c.timer.begin(c.stream, "immct_conformation");
-        hipLaunchKernelGGL(k_immct_conformation, dim3(nbB), dim3(B), 0, c.stream, x, v, c.mat.p, rest, u.nbr.p,
+        const auto s_conf = shape("immct_conformation"), s_str = shape("immct_stress_acc");
+        hipLaunchKernelGGL(k_immct_conformation, s_conf.first, s_conf.second, 0, c.stream, x, v, c.mat.p, rest, u.nbr.p,
                            u.cnt.p, n, st, q, c.fpoly.p, c.Q[0].p, c.Lgrad.p, c.stress.p, c.vis_ct.p, c.srec.p, c.scal.p);
         c.timer.end(c.stream);
         c.timer.begin(c.stream, "immct_stress_acc");
-        hipLaunchKernelGGL(k_immct_stress_acc<MP>, dim3(nbB), dim3(B), 0, c.stream, x, v, c.vadv.p, c.mat.p, rest,
+        hipLaunchKernelGGL(k_immct_stress_acc<MP>, s_str.first, s_str.second, 0, c.stream, x, v, c.vadv.p, c.mat.p, rest,
                            c.cd.p, c.fpoly.p, c.srec.p, u.nbr.p, u.cnt.p, n, st, q, ph, c.scal.p);
         c.timer.end(c.stream);
     }
@@ -914,22 +921,37 @@ static void imm_step_impl(Ctx &c) {
     const uint32_t cap = c.p.max_solver_iterations;
     hipLaunchKernelGGL(k_phase_begin, dim3(1), dim3(64), 0, c.stream, c.status.p);
     c.timer.begin(c.stream, "imm_fluxes");
-    hipLaunchKernelGGL(k_imm_fluxes<MP>, dim3(nbB), dim3(B), 0, c.stream, x, v, c.mat.p, c.flag.p, u.nbr.p, u.cnt.p,
-                       n, st, q, ph, c.status.p, c.scal.p, 0);
+    hipLaunchKernelGGL(k_imm_fluxes<MP>, s_flux.first, s_flux.second, 0, c.stream, x, v, c.mat.p, c.flag.p, u.nbr.p,
+                       u.cnt.p, n, st, q, ph, c.status.p, c.scal.p, 0);
     c.timer.end(c.stream);
-    uint32_t chunk = 2, launched = 0;
+    // Rounds are enqueued in chunks as in the pressure solves: the previous
+    // tick's round count first, then max(1, hint/4) per host check.  Loop
+    // iteration k checks round k + 1 and, if a fraction went negative,
+    // recomputes the fluxes (round k + 2); with R rounds the checks 0..R-1
+    // and the recomputations 0..R-2 did work, and only those stay timed.
+    const uint32_t phint = c.iter_hint_phase;
+    const uint32_t pstep = c.opt_chunk ? c.opt_chunk : std::max(1u, phint / 4);
+    uint32_t chunk = c.opt_chunk ? c.opt_chunk : std::max(1u, phint), launched = 0;
+    const size_t tmark = c.timer.mark();
     for (;;) {
         for (uint32_t it = 0; it < chunk && launched <= cap; ++it, ++launched) {
+            c.timer.begin(c.stream, "imm_check_negative");
             hipLaunchKernelGGL(k_imm_check_negative<MP>, dim3(nb), dim3(256), 0, c.stream, c.mat.p, c.flag.p, n, q,
                                ph, c.status.p);
             hipLaunchKernelGGL(k_imm_phase_decide, dim3(1), dim3(64), 0, c.stream, c.status.p, cap);
-            hipLaunchKernelGGL(k_imm_fluxes<MP>, dim3(nbB), dim3(B), 0, c.stream, x, v, c.mat.p, c.flag.p, u.nbr.p,
-                               u.cnt.p, n, st, q, ph, c.status.p, c.scal.p, c.opt_full_flux_retry ? 0 : 1);
+            c.timer.end(c.stream);
+            c.timer.begin(c.stream, "imm_fluxes");
+            hipLaunchKernelGGL(k_imm_fluxes<MP>, s_flux.first, s_flux.second, 0, c.stream, x, v, c.mat.p, c.flag.p,
+                               u.nbr.p, u.cnt.p, n, st, q, ph, c.status.p, c.scal.p, c.opt_full_flux_retry ? 0 : 1);
+            c.timer.end(c.stream);
         }
         read_status(c);
         if (c.h_status[kStDone] || launched > cap) break;
-        chunk = std::min(chunk * 2, cap + 1);
+        chunk = pstep;
     }
+    const uint32_t rounds = std::max(1u, c.h_status[kStPhaseIter]);
+    if (c.timer.enabled) c.timer.drop_from(tmark + 2 * (size_t)(rounds - 1) + 1);
+    c.iter_hint_phase = rounds;
     c.timer.begin(c.stream, "imm_phase_update");
     hipLaunchKernelGGL(k_imm_phase_update<MP>, dim3(nb), dim3(256), 0, c.stream, x, v, c.vadv.p, c.mat.p, c.flag.p,
                        rest, c.color4[0].p, n, q, ph);

@@ -310,13 +310,19 @@ static void pbf_enqueue_step(Ctx &c, PbfTapeStep *tape = nullptr) {
                        u.nbr.p, u.cnt.p, n, u.stride, q, tape ? 1 : 0)
         if (it > 0 || !build_lambda) {
             c.timer.begin(c.stream, "pbf_lambda");
-            if (q.mass_weighted) { if (L == 4) VTP_LAMBDA(true, 4); else VTP_LAMBDA(true, 1); }
-            else { if (L == 4) VTP_LAMBDA(false, 4); else VTP_LAMBDA(false, 1); }
+            if (q.mass_weighted) {
+                if (L == 4) VTP_LAMBDA(true, 4); else if (L == 2) VTP_LAMBDA(true, 2); else VTP_LAMBDA(true, 1);
+            } else {
+                if (L == 4) VTP_LAMBDA(false, 4); else if (L == 2) VTP_LAMBDA(false, 2); else VTP_LAMBDA(false, 1);
+            }
             c.timer.end(c.stream);
         }
         c.timer.begin(c.stream, "pbf_dx");
-        if (q.mass_weighted) { if (L == 4) VTP_DX(true, 4); else VTP_DX(true, 1); }
-        else { if (L == 4) VTP_DX(false, 4); else VTP_DX(false, 1); }
+        if (q.mass_weighted) {
+            if (L == 4) VTP_DX(true, 4); else if (L == 2) VTP_DX(true, 2); else VTP_DX(true, 1);
+        } else {
+            if (L == 4) VTP_DX(false, 4); else if (L == 2) VTP_DX(false, 2); else VTP_DX(false, 1);
+        }
 #undef VTP_LAMBDA
 #undef VTP_DX
         c.timer.end(c.stream);

@@ -141,9 +141,23 @@ extern "C" vtp_status vtp_state_size(vtp_ctx *ctx, size_t *bytes) {
     return VTP_OK;
 }
 
+namespace vtp {
+size_t state_image_bytes(Ctx &c) { return sizeof(vtp_state_header) + payload_bytes(c); }
+vtp_status save_state_image(Ctx &c, void *host_dst, size_t bytes);
+vtp_status load_state_image(Ctx &c, const void *host_src, size_t bytes);
+}  // namespace vtp
+
 extern "C" vtp_status vtp_save_state(vtp_ctx *ctx, void *host_dst, size_t bytes) {
     if (!ctx) return VTP_EINVAL;
-    Ctx &c = ctx->c;
+    return save_state_image(ctx->c, host_dst, bytes);
+}
+
+extern "C" vtp_status vtp_load_state(vtp_ctx *ctx, const void *host_src, size_t bytes) {
+    if (!ctx) return VTP_EINVAL;
+    return load_state_image(ctx->c, host_src, bytes);
+}
+
+vtp_status vtp::save_state_image(Ctx &c, void *host_dst, size_t bytes) {
     if (!c.initialized) return fail(c, VTP_ESTATE, "vtp_save_state: not initialized");
     if (!host_dst) return fail(c, VTP_EINVAL, "vtp_save_state: null destination");
     const size_t need = sizeof(vtp_state_header) + payload_bytes(c);
@@ -181,9 +195,7 @@ extern "C" vtp_status vtp_save_state(vtp_ctx *ctx, void *host_dst, size_t bytes)
     });
 }
 
-extern "C" vtp_status vtp_load_state(vtp_ctx *ctx, const void *host_src, size_t bytes) {
-    if (!ctx) return VTP_EINVAL;
-    Ctx &c = ctx->c;
+vtp_status vtp::load_state_image(Ctx &c, const void *host_src, size_t bytes) {
     if (!c.initialized) return fail(c, VTP_ESTATE, "vtp_load_state: not initialized");
     if (!host_src || bytes < sizeof(vtp_state_header)) return fail(c, VTP_EINVAL, "vtp_load_state: no header");
     vtp_state_header h;

@@ -1,75 +1,131 @@
-// tune.hip - launch-shape autotuner for the neighbour passes.
+// tune.hip - launch-shape autotuner for the neighbour passes of every solver.
 //
-// For every candidate workgroup size the tuner runs two ticks from the same
-// saved state with hipEvent stage timing, keeps the fastest size per stage,
-// and restores the state, so tuning does not advance or perturb the
-// simulation.  PBF and DFSPH only (their state is X, V and the slot map).
+// Each candidate shape runs two ticks from the same state image, the first
+// untimed (warm-up at that shape), the second with the stage hipEvents, and
+// the image is loaded back after every candidate: tuning neither advances
+// nor perturbs the simulation (positions, velocities, solver fields, time,
+// step and iteration counters, crash and cap words are the image's again,
+// and the lists are stale as after any load, so nothing the probe ticks
+// computed stays readable).
+//
+// Mode 1: the workgroup size of each neighbour pass, from {64, 128, 256}.
+// A particle's sums do not depend on it, so results stay bit-identical.
+// Mode 2: also 1, 2 or 4 lanes per particle in the PBF passes (with more
+// than one lane the first constraint iteration is not folded into the
+// neighbour build), chosen by the summed time of the build and the passes.
+// More lanes change the order of a particle's neighbour sums, so results
+// then agree with mode 1 within FP32 rounding, not bit for bit.
 #include "vtp_internal.h"
 
 namespace vtp {
 
-void autotune(Ctx &c) {
-    if (!c.initialized) return;
-    if (c.p.solver != VTP_PBF && c.p.solver != VTP_DFSPH) return;
-    const uint32_t n = c.n;
-    const std::vector<std::string> stages =
-        c.p.solver == VTP_PBF
-            ? std::vector<std::string>{"pbf_lambda", "pbf_dx", "pbf_xsph"}
-            : std::vector<std::string>{"df_precompute", "df_delta_kappa", "df_update_vadv", "df_vis_normal"};
-    // timer names of DFSPH loop stages carry the solve prefix
-    auto timed_name = [&](const std::string &s) -> std::vector<std::string> {
-        if (s == "df_delta_kappa") return {"df_div_delta_kappa", "df_inc_delta_kappa"};
-        if (s == "df_update_vadv") return {"df_div_update_vadv", "df_inc_update_vadv"};
-        if (s == "df_vis_normal") return {"df_vis_normal", "df_tension_apply"};
-        return {s};
-    };
-    DevBuf<float4> sx, sv;
-    DevBuf<uint32_t> so;
-    DevBuf<uint32_t> sstat;
-    sx.alloc(n);
-    sv.alloc(n);
-    so.alloc(n);
-    sstat.alloc(kStWords);
-    VTP_HIP(hipMemcpyAsync(sx.p, c.X[0].p, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
-    VTP_HIP(hipMemcpyAsync(sv.p, c.V[0].p, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
-    VTP_HIP(hipMemcpyAsync(so.p, c.orig.p, n * 4, hipMemcpyDeviceToDevice, c.stream));
-    VTP_HIP(hipMemcpyAsync(sstat.p, c.status.p, kStWords * 4, hipMemcpyDeviceToDevice, c.stream));
-    const uint64_t total_div = c.total_div, total_inc = c.total_inc;
-    const uint32_t hint_div = c.iter_hint_div, hint_inc = c.iter_hint_inc;
-    const bool timing_was = c.timer.enabled;
-    const bool graph_was = c.opt_graph;
-    c.opt_graph = false;
-    std::map<std::string, std::pair<double, uint32_t>> best;
-    const uint32_t cand[] = {64, 128, 256};   // the passes are built for at most 256 threads
-    for (uint32_t b : cand) {
-        for (const auto &s : stages) c.block_of[s] = b;
-        c.timer.reset();
-        c.timer.enabled = false;
-        solver_tick(c);   // warm-up at this shape
-        c.timer.enabled = true;
-        solver_tick(c);
-        c.timer.collect();
-        for (const auto &s : stages) {
-            double t = 0;
-            for (const auto &nm : timed_name(s)) t += c.timer.ms.count(nm) ? c.timer.ms[nm] : 0.0;
-            if (!best.count(s) || t < best[s].first) best[s] = {t, b};
+namespace {
+
+struct Pass {
+    const char *block;                // key of Ctx::block_of
+    std::vector<const char *> timed;  // stage-timer names of its launches
+};
+
+std::vector<Pass> passes_of(const Ctx &c) {
+    const Pass dk{"df_delta_kappa", {"df_div_delta_kappa", "df_inc_delta_kappa"}};
+    const Pass up{"df_update_vadv", {"df_div_update_vadv", "df_inc_update_vadv"}};
+    switch (c.p.solver) {
+        case VTP_PBF:
+            return {{"pbf_lambda", {"pbf_lambda"}}, {"pbf_dx", {"pbf_dx"}}, {"pbf_xsph", {"pbf_xsph"}}};
+        case VTP_DFSPH:
+            return {{"df_precompute", {"df_precompute"}}, dk, up, {"df_vis_normal", {"df_vis_normal"}}};
+        default: {
+            std::vector<Pass> v{dk, up, {"imm_precompute", {"imm_precompute"}},
+                                {"imm_vis_normal", {"imm_vis_normal"}},
+                                {"imm_tension_phase_vel", {"imm_tension_phase_vel"}},
+                                {"imm_fluxes", {"imm_fluxes"}}};
+            if (c.p.solver == VTP_IMMCT) {
+                v.push_back({"immct_conformation", {"immct_conformation"}});
+                v.push_back({"immct_stress_acc", {"immct_stress_acc"}});
+            }
+            return v;
         }
-        // back to the saved state
-        VTP_HIP(hipMemcpyAsync(c.X[0].p, sx.p, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
-        VTP_HIP(hipMemcpyAsync(c.V[0].p, sv.p, n * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
-        VTP_HIP(hipMemcpyAsync(c.orig.p, so.p, n * 4, hipMemcpyDeviceToDevice, c.stream));
-        VTP_HIP(hipMemcpyAsync(c.status.p, sstat.p, kStWords * 4, hipMemcpyDeviceToDevice, c.stream));
-        void launch_mat_from_orig(Ctx &);
-        launch_mat_from_orig(c);
     }
-    for (const auto &kv : best) c.block_of[kv.first] = kv.second.second;
-    c.total_div = total_div;
-    c.total_inc = total_inc;
-    c.iter_hint_div = hint_div;
-    c.iter_hint_inc = hint_inc;
+}
+
+double timed_ms(const StageTimer &t, const std::vector<const char *> &names) {
+    double ms = 0;
+    for (const char *nm : names) {
+        auto it = t.ms.find(nm);
+        if (it != t.ms.end()) ms += it->second;
+    }
+    return ms;
+}
+
+}  // namespace
+
+void autotune(Ctx &c, int mode) {
+    if (!c.initialized) return;
+    const std::vector<Pass> passes = passes_of(c);
+    std::vector<unsigned char> image(state_image_bytes(c));
+    if (save_state_image(c, image.data(), image.size()) != VTP_OK)
+        throw std::runtime_error("autotune: state image: " + c.err);
+    const bool timing_was = c.timer.enabled, graph_was = c.opt_graph;
+    const uint32_t lanes_was = c.opt_lanes, hint_phase = c.iter_hint_phase;
+    const std::map<std::string, uint32_t> blocks_was = c.block_of;
+    c.opt_graph = false;
+    c.timer.reset();
+    const uint32_t cand_block[] = {64, 128, 256};   // the passes are built for at most 256 threads
+    std::vector<uint32_t> cand_lanes{1};
+    if (mode == 2 && c.p.solver == VTP_PBF) cand_lanes = {1, 2, 4};
+    // per lane count: the best block of each pass and the summed time
+    double best_total = -1;
+    uint32_t best_lanes = c.opt_lanes;
+    std::map<std::string, uint32_t> best_blocks;
+    std::string failure;
+    try {
+        for (uint32_t L : cand_lanes) {
+            c.opt_lanes = L;
+            std::map<std::string, std::pair<double, uint32_t>> best;
+            double build_ms = 0;
+            for (uint32_t b : cand_block) {
+                for (const Pass &p : passes) c.block_of[p.block] = b;
+                c.timer.reset();
+                c.timer.enabled = false;
+                solver_tick(c);   // warm-up at this shape
+                c.timer.enabled = true;
+                solver_tick(c);
+                c.timer.collect();
+                for (const Pass &p : passes) {
+                    const double t = timed_ms(c.timer, p.timed);
+                    auto it = best.find(p.block);
+                    if (it == best.end() || t < it->second.first) best[p.block] = {t, b};
+                }
+                build_ms += timed_ms(c.timer, {"ugns_neighbors"}) / 3.0;
+                c.timer.enabled = false;
+                if (load_state_image(c, image.data(), image.size()) != VTP_OK)
+                    throw std::runtime_error("autotune: restoring the state image: " + c.err);
+            }
+            double total = build_ms;
+            for (const auto &kv : best) total += kv.second.first;
+            if (best_total < 0 || total < best_total) {
+                best_total = total;
+                best_lanes = L;
+                best_blocks.clear();
+                for (const auto &kv : best) best_blocks[kv.first] = kv.second.second;
+            }
+        }
+    } catch (const std::exception &e) {
+        failure = e.what();
+    }
     c.timer.reset();
     c.timer.enabled = timing_was;
     c.opt_graph = graph_was;
+    c.iter_hint_phase = hint_phase;
+    if (!failure.empty()) {
+        c.block_of = blocks_was;
+        c.opt_lanes = lanes_was;
+        throw std::runtime_error(failure);
+    }
+    c.block_of = blocks_was;
+    for (const auto &kv : best_blocks) c.block_of[kv.first] = kv.second;
+    c.opt_lanes = best_lanes;
+    pbf_capture_reset(c);   // a captured tick holds the old launch shapes
     VTP_HIP(hipStreamSynchronize(c.stream));
 }
 

@@ -98,6 +98,17 @@ struct StageTimer {
         }
         pending.clear();
     }
+    // Solver loops enqueue iterations past their exit (they return at once);
+    // a loop marks the pending list before it and drops what followed the
+    // last iteration that did work, so that only working launches are timed.
+    size_t mark() const { return pending.size(); }
+    void drop_from(size_t m) {
+        for (size_t k = m; k < pending.size(); ++k) {
+            pool.push_back(pending[k].a);
+            if (pending[k].b) pool.push_back(pending[k].b);
+        }
+        if (m < pending.size()) pending.resize(m);
+    }
     void reset() { collect(); ms.clear(); launches.clear(); order.clear(); }
     ~StageTimer() {
         for (auto &p : pending) { (void)hipEventDestroy(p.a); if (p.b) (void)hipEventDestroy(p.b); }
@@ -263,7 +274,11 @@ void dfsph_initialize(Ctx &c);
 void dfsph_step(Ctx &c);
 void imm_initialize(Ctx &c);
 void imm_step(Ctx &c);
-void autotune(Ctx &c);
+void autotune(Ctx &c, int mode);
+// state images (the vtp_save_state / vtp_load_state layout), host memory
+size_t state_image_bytes(Ctx &c);
+vtp_status save_state_image(Ctx &c, void *host_dst, size_t bytes);
+vtp_status load_state_image(Ctx &c, const void *host_src, size_t bytes);
 void solver_tick(Ctx &c);
 
 // Solver context: everything one vtp_ctx owns.
@@ -292,10 +307,10 @@ struct Ctx {
 
     // options
     bool opt_graph = false;
-    bool opt_autotune = false;
+    int opt_autotune = 0;     // 0 off, 1 workgroup sizes, 2 also PBF lanes per particle
     uint32_t opt_block = 256;
     uint32_t opt_chunk = 0;   // 0 = adaptive
-    uint32_t opt_lanes = 1;   // lanes per particle in the PBF neighbour passes (1 or 4)
+    uint32_t opt_lanes = 1;   // lanes per particle in the PBF neighbour passes (1, 2 or 4)
     bool opt_static_lists = false;   // PBF ticks also build the (unread) lists of static particles
     bool opt_full_flux_retry = false;  // IMM: every negative-fraction retry recomputes all fluxes
     bool opt_build_lambda = true;   // PBF: the list build also computes the first lambda pass of a tick
@@ -334,7 +349,7 @@ struct Ctx {
     DevBuf<int32_t> obj_fluid;               // 1 if the object is fluid
     DevBuf<double> obj_w;                    // residual weight per object
     DevBuf<double> partial;                  // residual partial per workgroup
-    uint32_t iter_hint_div = 1, iter_hint_inc = 1;
+    uint32_t iter_hint_div = 1, iter_hint_inc = 1, iter_hint_phase = 2;
     DevBuf<float4> vadv, acc, normal;
     DevBuf<float4> xk;   // (x, V kappa) of the current solver iteration, gathered by the v* pass
     DevBuf<float4> g1;   // sum_j V_j gradW_ij of the tick (positions are fixed during the solves)

@@ -173,6 +173,8 @@ SIGNATURES = {
     "vtp_particle_count": (C.c_uint32, [_P]),
     "vtp_read": (C.c_int, [_P, C.c_int32, C.c_void_p, C.c_size_t]),
     "vtp_write": (C.c_int, [_P, C.c_int32, C.c_void_p, C.c_size_t]),
+    "vtp_read_device": (C.c_int, [_P, C.c_int32, C.c_void_p, C.c_size_t]),
+    "vtp_write_device": (C.c_int, [_P, C.c_int32, C.c_void_p, C.c_size_t]),
     "vtp_stats": (C.c_int, [_P, C.POINTER(VtpStepStats)]),
     "vtp_debug_neighbors": (C.c_int, [_P, _U, _U, _U, _U, _U]),
     "vtp_grid_info": (C.c_int, [_P, _U, _U]),
@@ -180,11 +182,14 @@ SIGNATURES = {
     "vtp_save_state": (C.c_int, [_P, C.c_void_p, C.c_size_t]),
     "vtp_load_state": (C.c_int, [_P, C.c_void_p, C.c_size_t]),
     "vtp_set_option": (C.c_int, [_P, C.c_char_p, C.c_double]),
+    "vtp_get_option": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double)]),
     "vtp_stage_times": (C.c_int32, [_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_double),
                                     C.POINTER(C.c_uint64), C.c_int32]),
     "vtp_reset_stage_times": (C.c_int, [_P]),
     "vtp_pbf_rollout_fwd": (C.c_int, [_P, C.c_uint32, C.c_uint32]),
     "vtp_pbf_rollout_bwd": (C.c_int, [_P, _F, _F, _F, _F, _F]),
+    "vtp_pbf_rollout_bwd_device": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "vtp_pbf_density_loss_device": (C.c_int, [_P, C.c_void_p, C.c_void_p]),
     "vtp_pbf_density_loss": (C.c_int, [_P, C.POINTER(C.c_double), _F]),
     "vtp_pbf_density_loss_noalloc": (C.c_int, [_P, C.c_void_p, C.c_void_p]),
     "vtp_pbf_set_xsph": (C.c_int, [_P, C.c_float]),
@@ -329,6 +334,24 @@ class Context:
     def set_option(self, key: str, value: float):
         self._check(self.lib.vtp_set_option(self.h, key.encode(), float(value)))
 
+    def get_option(self, key: str) -> float:
+        v = C.c_double()
+        self._check(self.lib.vtp_get_option(self.h, key.encode(), C.byref(v)))
+        return v.value
+
+    TUNED_PASSES = {0: ("pbf_lambda", "pbf_dx", "pbf_xsph"),
+                    1: ("df_precompute", "df_delta_kappa", "df_update_vadv", "df_vis_normal"),
+                    2: ("df_delta_kappa", "df_update_vadv", "imm_precompute", "imm_vis_normal",
+                        "imm_tension_phase_vel", "imm_fluxes"),
+                    3: ("df_delta_kappa", "df_update_vadv", "imm_precompute", "imm_vis_normal",
+                        "imm_tension_phase_vel", "imm_fluxes", "immct_conformation", "immct_stress_acc")}
+
+    def launch_shapes(self, solver_id: int) -> dict:
+        """Workgroup size of every neighbour pass of the solver, and the PBF lanes."""
+        d = {p: int(self.get_option("block." + p)) for p in self.TUNED_PASSES[solver_id]}
+        d["lanes"] = int(self.get_option("lanes"))
+        return d
+
     def _shape(self, field: str):
         fid, comps = FIELDS[field]
         n = self.lib.vtp_particle_count(self.h)
@@ -348,6 +371,17 @@ class Context:
         fid, shape = self._shape(field)
         a = np.ascontiguousarray(arr, dtype=np.float32).reshape(shape)
         self._check(self.lib.vtp_write(self.h, fid, a.ctypes.data_as(C.c_void_p), a.nbytes))
+
+    # device-pointer variants: `ptr` is the address of a float[n][3] array in
+    # attach order on the context's GPU (e.g. torch's data_ptr()); the caller
+    # has finished writing it, and the call returns after the context's stream
+    def read_device(self, field: str, ptr: int):
+        fid, shape = self._shape(field)
+        self._check(self.lib.vtp_read_device(self.h, fid, ptr, 12 * shape[0]))
+
+    def write_device(self, field: str, ptr: int):
+        fid, shape = self._shape(field)
+        self._check(self.lib.vtp_write_device(self.h, fid, ptr, 12 * shape[0]))
 
     def stats(self) -> dict:
         s = VtpStepStats()
@@ -424,6 +458,15 @@ class Context:
         g = np.zeros((n, 3), np.float32) if with_grad else None
         self._check(self.lib.vtp_pbf_density_loss(self.h, C.byref(loss), fptr(g)))
         return loss.value, g
+
+    def rollout_bwd_device(self, gx_ptr, gv_ptr, gv0_ptr, gx0_ptr, gk_ptr):
+        """vtp_pbf_rollout_bwd on device pointers (0 or None for an absent array)."""
+        self._check(self.lib.vtp_pbf_rollout_bwd_device(self.h, gx_ptr or None, gv_ptr or None, gv0_ptr or None,
+                                                        gx0_ptr or None, gk_ptr or None))
+
+    def density_loss_device(self, loss_ptr, g_ptr=None):
+        """vtp_pbf_density_loss with the loss (one double) and dL/dx in device memory."""
+        self._check(self.lib.vtp_pbf_density_loss_device(self.h, loss_ptr, g_ptr or None))
 
     def set_xsph(self, k: float):
         self._check(self.lib.vtp_pbf_set_xsph(self.h, float(k)))
