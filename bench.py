@@ -32,25 +32,26 @@ def pbf_step_bytes(n, nf, cells, K, passes):
     return nf * (48 + 88 + 16 * passes) + 8 * cells + K * (32 * n + 16 * nf) + 28 * n + 24 * nf
 
 
-def kernel_bytes(name, n, nf):
+def kernel_bytes(name, n, nf, fold=False):
     """Algorithmic bytes per launch of the PBF neighbour passes, split from the
     per-tick formula above: an iteration's 32 N + 16 N_f is the density+lambda
     pass (x, m in -> lambda out: 16 N + 4 N_f) plus the correction pass
     (x, lambda in -> x out: 16 N + 12 N_f); XSPH + apply is 28 N + 24 N_f.
-    Neighbour re-reads and lists are not credited.  The build is credited
-    with the first iteration's density+lambda pass it computes."""
+    Neighbour re-reads and lists are not credited, so the list build has no
+    share of its own; with `fold` (option build_lambda) it is credited with
+    the first iteration's density+lambda pass it computes."""
     if name == "pbf_lambda":
         return 16 * n + 4 * nf
     if name == "pbf_dx":
         return 16 * n + 12 * nf
     if name == "pbf_xsph":
         return 28 * n + 24 * nf
-    if name == "ugns_neighbors":
+    if name == "ugns_neighbors" and fold:
         return 16 * n + 4 * nf
     return None
 
 
-def kernel_bytes_own_fields(name, n, nf):
+def kernel_bytes_own_fields(name, n, nf, fold=False):
     """Bytes per launch counting every field the kernel itself reads and
     writes once (material words, both halves of the (x, lambda) records,
     velocities), the per-kernel count of the design notes."""
@@ -61,7 +62,8 @@ def kernel_bytes_own_fields(name, n, nf):
     if name == "pbf_xsph":        # x, v, mat in (all); x, v out (all)
         return 68 * n
     if name == "ugns_neighbors":  # x + key in; count word out; with the tick's first lambda pass
-        return 36 * n + 8 * nf    # mat in, lambda out (all), rho and C out (fluid)
+        # mat in, lambda out (all), rho and C out (fluid)
+        return 36 * n + 8 * nf if fold else 24 * n
     return None
 
 
@@ -89,6 +91,9 @@ def build_ctx(vtp, wl, graph=True):
     for o in wl.objects:
         ctx.add_particles(o.pos, o.mat, o.vel, o.vol_frac, o.scene)
     ctx.set_option("graph", 1 if graph else 0)
+    # workgroup size of every neighbour pass timed at initialize (results
+    # bit-identical to any other choice)
+    ctx.set_option("autotune", 1)
     ctx.initialize()
     return ctx, params
 
@@ -126,6 +131,7 @@ def bench_dfsph(torch, vtp, scenes, steps, warmup):
     el = timed_steps(torch, ctx, steps)
     st = ctx.stats()
     stages = stage_breakdown(ctx, 2)
+    shapes = ctx.launch_shapes(vtp._abi.DFSPH)
     ctx.close()
     it_div = (st["total_div_iterations"] - st0["total_div_iterations"]) / steps
     it_inc = (st["total_incomp_iterations"] - st0["total_incomp_iterations"]) / steps
@@ -136,7 +142,7 @@ def bench_dfsph(torch, vtp, scenes, steps, warmup):
             "ms_per_step": round(ms, 4), "mean_div_iterations": it_div,
             "mean_incomp_iterations": it_inc, "steps": steps, "warmup": warmup,
             "step_algorithmic_bytes": int(sb), "step_hbm_frac": round(sb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
-            "dead_particles": st["dead_particles"], "stages_ms_per_step": stages}
+            "dead_particles": st["dead_particles"], "stages_ms_per_step": stages, "launch_shapes": shapes}
 
 
 def bench_immct(torch, vtp, scenes, steps, warmup):
@@ -151,6 +157,7 @@ def bench_immct(torch, vtp, scenes, steps, warmup):
     el = timed_steps(torch, ctx, steps)
     st = ctx.stats()
     stages = stage_breakdown(ctx, 1)
+    shapes = ctx.launch_shapes(vtp._abi.IMMCT)
     ctx.close()
     it_div = (st["total_div_iterations"] - st0["total_div_iterations"]) / steps
     it_inc = (st["total_incomp_iterations"] - st0["total_incomp_iterations"]) / steps
@@ -164,7 +171,7 @@ def bench_immct(torch, vtp, scenes, steps, warmup):
             "ms_per_step": round(ms, 4), "steps": steps, "warmup": warmup,
             "mean_div_iterations": it_div, "mean_incomp_iterations": it_inc,
             "step_algorithmic_bytes": int(sb), "step_hbm_frac": round(sb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
-            "mass_weighted_stress": True, "last_iterations": [st["div_iterations"], st["incomp_iterations"],
+            "mass_weighted_stress": True, "launch_shapes": shapes, "last_iterations": [st["div_iterations"], st["incomp_iterations"],
                                                               st["phase_iterations"]]}
 
 
@@ -184,6 +191,7 @@ def bench_batched(torch, vtp, scenes, steps, warmup):
         for o in objs:
             c.add_particles(o.pos, o.mat, o.vel, None, 0)
         c.set_option("graph", 1)
+        c.set_option("autotune", 1)
         c.initialize()
         c.step(warmup)
         ctxs.append(c)
@@ -202,6 +210,16 @@ def bench_batched(torch, vtp, scenes, steps, warmup):
             c.step(1)
     torch.cuda.synchronize()
     t_inter = time.perf_counter() - t0
+    # the per-scene ticks as plain launch sequences (no graph replay)
+    for c in ctxs:
+        c.set_option("graph", 0)
+        c.step(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for c in ctxs:
+        c.step(steps)
+    torch.cuda.synchronize()
+    t_eager = time.perf_counter() - t0
     for c in ctxs:
         c.close()
     return {"workload": wl.name, "scenes": len(per), "fluid_particles": wl.n_fluid,
@@ -211,6 +229,8 @@ def bench_batched(torch, vtp, scenes, steps, warmup):
             "gain": round(t_single / t_batched, 3),
             "one_context_per_scene_interleaved_ms_per_step": round(t_inter * 1e3 / steps, 4),
             "gain_interleaved": round(t_inter / t_batched, 3),
+            "one_context_per_scene_eager_ms_per_step": round(t_eager * 1e3 / steps, 4),
+            "gain_eager": round(t_eager / t_batched, 3),
             "batched_value": round(wl.n_fluid * steps / t_batched, 1), "unit": "particle-steps/s"}
 
 
@@ -450,14 +470,19 @@ def main():
     total = sum(ms for ms, _ in stages.values())
     dom = max(stages, key=lambda k: stages[k][0])
     dom_ms = stages[dom][0] / stages[dom][1]
-    kb = kernel_bytes(dom, n, nf)
+    fold = bool(ctx.get_option("build_lambda"))
+    kb = kernel_bytes(dom, n, nf, fold)
+    own = kernel_bytes_own_fields(dom, n, nf, fold)
     roof = None
-    if kb is not None:
-        ach = kb / (dom_ms * 1e-3) / 1e9
-        own = kernel_bytes_own_fields(dom, n, nf)
+    if own is not None:
+        # a kernel with no share in the per-tick formula (the list build
+        # without the folded lambda pass) is rated on its own fields
+        basis = kb if kb is not None else own
+        ach = basis / (dom_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
-                "kernel_ms": round(dom_ms, 5), "algorithmic_bytes": kb,
+                "kernel_ms": round(dom_ms, 5), "algorithmic_bytes": kb if kb is not None else 0,
+                "bytes_basis": "per-tick formula" if kb is not None else "own fields",
                 "own_field_bytes": own, "own_field_frac": round(own / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
         if TRAFFIC_JSON.exists():   # counter-measured HBM bytes per launch (rocprofv3 --pmc, calibrated)
             t = json.loads(TRAFFIC_JSON.read_text())["kernels"].get(dom)
@@ -473,14 +498,16 @@ def main():
         if name not in stages or not stages[name][1]:
             continue
         ms = stages[name][0] / stages[name][1]
-        ab = kernel_bytes(name, n, nf)
+        ab = kernel_bytes(name, n, nf, fold)
         e = {"us_per_launch": round(ms * 1e3, 2), "launches_per_tick": round(stages[name][1] / args.steps, 2),
-             "frac": round(ab / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
+             "frac": round(ab / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5) if ab is not None else None,
+             "own_field_frac": round(kernel_bytes_own_fields(name, n, nf, fold) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
         if name in traffic:
             e["traffic_frac"] = round(traffic[name]["bytes"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)
         roof_all[name] = e
     passes = key_passes(cells * wl.scenes())
     step_bytes = pbf_step_bytes(n, nf, cells * wl.scenes(), params.iteration_num, passes)
+    shapes = ctx.launch_shapes(vtp._abi.PBF)   # the tuner's choice
     st = ctx.stats()
     import numpy as np
     # ticks build fluid lists only; one more tick with every list for the statistics
@@ -514,7 +541,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": wl.name, "fluid_particles": nf, "boundary_particles": n - nf,
                    "grid": list(grid), "iterations": params.iteration_num, "dt": params.time_step,
-                   "graph": True},
+                   "graph": True, "build_lambda": fold, "launch_shapes": shapes},
         "roofline": roof,
         "roofline_by_kernel": roof_all,
         "cpu_baseline": cpu,

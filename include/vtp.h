@@ -335,9 +335,9 @@ vtp_status vtp_load_state(vtp_ctx *ctx, const void *host_src, size_t bytes);
  * "full_flux_retry" (IMM / IMM-CT: 0 = default, a negative-fraction retry
  * recomputes the fluxes of the particles next to newly flagged ones; 1 =
  * every retry recomputes all fluxes, as the reference.  Bit-identical),
- * "build_lambda" (PBF with "lanes" 1: 1 = default, the neighbour build also
+ * "build_lambda" (PBF with "lanes" 1: 1 = the neighbour build also
  * computes the first constraint iteration's density and lambda from the
- * neighbours it lists; 0 = a separate pass does.  Bit-identical). */
+ * neighbours it lists; 0 = default, a separate pass does.  Bit-identical). */
 vtp_status vtp_set_option(vtp_ctx *ctx, const char *key, double value);
 /* Reads an option back: the keys above except "stage_timing" and
  * "solver_chunk", and "block.<pass>" for the workgroup size a neighbour pass

@@ -136,9 +136,9 @@ def test_pbf_counting_and_radix_sort_bit_identical():
 
 @pytest.mark.parametrize("mass_weighted,walk", [(False, 0), (True, 0), (False, 2)])
 def test_pbf_lambda_from_the_build_bit_identical(mass_weighted, walk):
-    """By default the neighbour build computes the first iteration's density
-    and lambda pass while it writes each list, with the pass's per-neighbour
-    code in list order.  Ticks are bit-identical to running the pass: with
+    """With build_lambda the neighbour build computes the first iteration's
+    density and lambda pass while it writes each list, with the pass's
+    per-neighbour code in list order.  Ticks are bit-identical to running the pass: with
     particles that leave the grid (dead), lists cut by the cap (rebuilt in
     the reference's order), the reference walk, the mass-weighted gradient, a
     tick that also builds the static particles' lists, and a taped rollout
@@ -154,7 +154,7 @@ def test_pbf_lambda_from_the_build_bit_identical(mass_weighted, walk):
     # a crushed clump: lists past the cap in the first tick
     clump = (rng.uniform(-0.02, 0.02, size=(150, 3)) + [0.5, 0.05, 0.5]).astype(np.float32)
     wl.objects.append(scenes.Obj(vtp.MAT_FLUID, clump, None))
-    a, _ = _pair(wl, graph=1, neighbor_walk=walk)
+    a, _ = _pair(wl, graph=1, neighbor_walk=walk, build_lambda=1)
     b, _ = _pair(wl, graph=1, neighbor_walk=walk, build_lambda=0)
     fields = ("pos", "vel", "density", "lambda")
     dead = 0

@@ -313,7 +313,10 @@ struct Ctx {
     uint32_t opt_lanes = 1;   // lanes per particle in the PBF neighbour passes (1, 2 or 4)
     bool opt_static_lists = false;   // PBF ticks also build the (unread) lists of static particles
     bool opt_full_flux_retry = false;  // IMM: every negative-fraction retry recomputes all fluxes
-    bool opt_build_lambda = true;   // PBF: the list build also computes the first lambda pass of a tick
+    // PBF: the list build also computes the first lambda pass of a tick.  Off
+    // by default: with 16-bit lists a lambda launch is cheaper than what the
+    // sums cost the build (80 registers, 6 waves per SIMD against 8).
+    bool opt_build_lambda = false;
     StageTimer timer;
 
     // attach-order tables (static)
