@@ -106,9 +106,13 @@ def timed_steps(torch, ctx, steps):
     return time.perf_counter() - t0
 
 
-def stage_breakdown(ctx, steps):
+def stage_breakdown(ctx, steps, image=None):
     """Per-stage device ms per tick from hipEvents on the context's stream
-    (eager launches; run after the timed region)."""
+    (eager launches).  With `image` (the state image taken at the start of
+    the timed ticks) the same ticks are replayed, so the split covers the
+    timed work (a resume is bit-exact, iteration counts included)."""
+    if image is not None:
+        ctx.load_state(image)
     ctx.set_option("graph", 0)
     ctx.set_option("stage_timing", 1)
     ctx.reset_stage_times()
@@ -128,9 +132,10 @@ def bench_dfsph(torch, vtp, scenes, steps, warmup):
     _, cells = ctx.grid_info()
     ctx.step(warmup)
     st0 = ctx.stats()
+    image = ctx.save_state()
     el = timed_steps(torch, ctx, steps)
     st = ctx.stats()
-    stages = stage_breakdown(ctx, 2)
+    stages = stage_breakdown(ctx, steps, image)
     shapes = ctx.launch_shapes(vtp._abi.DFSPH)
     ctx.close()
     it_div = (st["total_div_iterations"] - st0["total_div_iterations"]) / steps
@@ -154,9 +159,10 @@ def bench_immct(torch, vtp, scenes, steps, warmup):
     _, cells = ctx.grid_info()
     ctx.step(warmup)
     st0 = ctx.stats()
+    image = ctx.save_state()
     el = timed_steps(torch, ctx, steps)
     st = ctx.stats()
-    stages = stage_breakdown(ctx, 1)
+    stages = stage_breakdown(ctx, steps, image)
     shapes = ctx.launch_shapes(vtp._abi.IMMCT)
     ctx.close()
     it_div = (st["total_div_iterations"] - st0["total_div_iterations"]) / steps

@@ -3,9 +3,12 @@
 Checks: iteration counts of both pressure solves equal the oracle's, exit
 residuals agree (FP64 device reduction vs FP64 host sum, rel 1e-4 since the
 per-particle deltas are FP32), and the fields after one step agree with the
-oracle within 1e-5 (L2 norm-wise relative; max norm 1e-5 for x, v, m, V).
-Derived quantities with cancellation (delta = max(0, cr - 1 + ...),
-kappa, acceleration = (v* - v)/dt) are compared at 1e-4.
+oracle within 1e-5 (L2 norm-wise relative; max norm 1e-5 for x, v, m, V),
+the acceleration (v* - v)/dt, the surface normal and the divergence solve's
+kappa included.  The density solve's predictor delta = max(0, cr - 1 + dt
+div v*) and its kappa are differences of numbers ~30x larger (cr ~ 1, delta
+~ 0.03 in this compressed block), so the FP32 rounding of cr (~1e-7 relative,
+measured 9.8e-8) reaches them amplified ~30-100x: 5e-5 (measured 1.1e-5).
 """
 import numpy as np
 import pytest
@@ -18,8 +21,8 @@ from vtp import scenes
 pytestmark = pytest.mark.gpu
 
 FIELD_TOL = {"pos": 1e-5, "vel": 1e-5, "mass": 1e-5, "volume": 1e-5, "rest_density": 1e-5,
-             "compression_ratio": 1e-5, "df_alpha": 1e-5, "vel_adv": 1e-5, "surface_normal": 1e-4,
-             "delta_cr": 1e-4, "kappa_incomp": 1e-4, "acc": 1e-4}
+             "compression_ratio": 1e-5, "df_alpha": 1e-5, "vel_adv": 1e-5, "surface_normal": 1e-5,
+             "kappa_div": 1e-5, "acc": 1e-5, "delta_cr": 5e-5, "kappa_incomp": 5e-5}
 
 
 def l2_rel(a, b):

@@ -5,8 +5,10 @@ Scene: three stacked one-hot phase slabs (densities 1000 / 800 / 1200) in a
 small box.  Checks after one tick: div / density / negative-fraction
 iteration counts equal the oracle's, and fields agree within 1e-5 (L2
 norm-wise relative) for positions, velocities, masses, volumes, rest
-density, fractions, phase velocities and colour; 1e-4 for quantities built
-from differences (velocity gradient, conformation tensor, stress, Cd).  The
+density, fractions, phase velocities and colour, and for the IMM-CT fields
+(velocity gradient, conformation tensor, stress, Cd, viscosity; the
+conformation tensor is held as Q - I on the device, so it keeps FP32
+precision near equilibrium).  The
 reference IMM-CT (no neighbour mass in the stress term) is also compared for
 one tick; its growth makes longer comparisons meaningless, so the multi-tick
 test uses the mass-weighted extension.
@@ -22,7 +24,7 @@ pytestmark = pytest.mark.gpu
 
 STATE = {"pos": 1e-5, "vel": 1e-5, "mass": 1e-5, "volume": 1e-5, "rest_density": 1e-5, "vol_frac": 1e-5,
          "vel_phase": 1e-5, "color": 1e-5}
-CT = {"Q": 1e-4, "vel_grad": 1e-4, "vise_stress": 1e-4, "cd": 1e-5, "vis": 1e-4}
+CT = {"Q": 1e-5, "vel_grad": 1e-5, "vise_stress": 1e-5, "cd": 1e-5, "vis": 1e-5}
 
 
 def l2_rel(a, b):
