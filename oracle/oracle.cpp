@@ -24,6 +24,20 @@
 //                      thinning term and the viscosity law's special points
 //                      (test_oracle_conformation.py).
 //   PBF adjoint      : central finite differences of this same FP64 forward.
+//   closed-form lattice sums (test_oracle_pins.py; sums over exact power-of-two
+//   lattices written from the reference kernels' formulas, each failing under
+//   a planted dropped factor, flipped sign or transposed index):
+//                      DFSPH alpha and the first kappa of a compressed block,
+//                      kappa and the v* change at a slab face; PBF delta-x of
+//                      a slab's top layer and XSPH under a quadratic shear;
+//                      the IMM drift flux of a linear drift and the diffusion
+//                      flux of a quadratic fraction field; IMM-CT velocity
+//                      gradient of a linear field, Cd on a uniform lattice,
+//                      the rheological factors at special fractions and the
+//                      polymer-stress acceleration at a slab face.
+// Every function above has a pin.  Parity unpinned: the reference's nvcc FP32
+// contraction choices (no CUDA toolchain here), which this FP64 restatement
+// does not model.
 // Values the reference leaves undefined (uninitialised boundary lambda and
 // kappa_incomp, racy boundary rest density) are defined as zero / race-free.
 #include "oracle.h"
