@@ -152,9 +152,12 @@ __global__ __launch_bounds__(256) void k_df_precompute(const float4 *__restrict_
     const uint32_t nn = w & ~kDeadBit;
     float c = (w & kDeadBit) ? 0.0f : vi.w * q.k.w0;
     float a1x = 0, a1y = 0, a1z = 0, a2 = 1e-6f;
-    struct D { float4 x; float V; int32_t m; };
+    // A neighbour is fluid exactly when its mass is not zero (fluid m =
+    // rho0 V0 > 0, boundary m = 0, as k_df_init sets them), so the material
+    // needs no gather of its own.
+    struct D { float4 x; float V; };
     for_each_neighbor(
-        nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg_w(v, j), ldg(mat, j)}; },
+        nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg_w(v, j)}; },
         [&](const D &d) {
             float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
             float W, f;
@@ -164,7 +167,7 @@ __global__ __launch_bounds__(256) void k_df_precompute(const float4 *__restrict_
             a1x = fmaf(d.V, gx, a1x);
             a1y = fmaf(d.V, gy, a1y);
             a1z = fmaf(d.V, gz, a1z);
-            if (d.m == VTP_MAT_FLUID) a2 += (gx * gx + gy * gy + gz * gz) * d.V * d.V / d.x.w;
+            if (d.x.w != 0.0f) a2 += (gx * gx + gy * gy + gz * gz) * d.V * d.V / d.x.w;
         });
     float a = (a1x * a1x + a1y * a1y + a1z * a1z) / xi.w + a2;
     cr[i] = c;
@@ -364,7 +367,6 @@ __global__ void k_df_apply_pressure(const float4 *__restrict__ x, float4 *__rest
 //   n_i = h sum_j m_j/rho0_j gradW_ij.
 __global__ __launch_bounds__(256) void k_df_vis_normal(const float4 *__restrict__ x, const float4 *__restrict__ v,
                                                        const int32_t *__restrict__ mat,
-                                                       const float *__restrict__ rest,
                                                        const float *__restrict__ vis,
                                                        const uint32_t *__restrict__ nbr,
                                                        const uint32_t *__restrict__ cnt, uint32_t n,
@@ -379,29 +381,31 @@ __global__ __launch_bounds__(256) void k_df_vis_normal(const float4 *__restrict_
     }
     const float4 xi = x[i];
     const float4 vi = v[i];
-    const float visi = vis[i];
     const uint32_t nn = cnt[i] & ~kDeadBit;
     float ax = q.gx, ay = q.gy, az = q.gz;
     float nx = 0, ny = 0, nz = 0;
-    struct D { float4 x, v; float mr, vis; int32_t m; };
+    // Same-material neighbours of a fluid particle are the fluid ones, those
+    // with non-zero mass (boundary m = 0), whose rest density is rho0_f and
+    // whose viscosity is the solver's one fPartRestViscosity (k_df_init sets
+    // every particle's), so a neighbour is its (x, m) and (v, V) records.
+    const float visi = vis[i];
+    const float vis_ij = 0.5f * (visi + visi);   // (mu_i + mu_j) / 2 with mu_j = mu_i
+    struct D { float4 x, v; };
     for_each_neighbor(
-        nbr, stride, i, nn,
-        [&](uint32_t j) {
-            float4 xj = ldg(x, j);
-            return D{xj, ldg(v, j), xj.w / ldg(rest, j), ldg(vis, j), ldg(mat, j)};
-        },
+        nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg(v, j)}; },
         [&](const D &d) {
-            if (d.m != mi) return;
+            if (d.x.w == 0.0f) return;
+            const float mr = d.x.w / q.rho0_f;
             float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
             float r2 = dx * dx + dy * dy + dz * dz;
             float f = cubic_grad_factor(r2, q.k);
             float vdx = (vi.x - d.v.x) * dx + (vi.y - d.v.y) * dy + (vi.z - d.v.z) * dz;
-            float pi = 0.5f * (visi + d.vis) * d.mr * vdx / (r2 + q.h2_001);
+            float pi = vis_ij * mr * vdx / (r2 + q.h2_001);
             float s = 10.0f * pi * f;
             ax = fmaf(dx, s, ax);
             ay = fmaf(dy, s, ay);
             az = fmaf(dz, s, az);
-            float sn = q.k.h * d.mr * f;
+            float sn = q.k.h * mr * f;
             nx = fmaf(dx, sn, nx);
             ny = fmaf(dy, sn, ny);
             nz = fmaf(dz, sn, nz);
@@ -433,14 +437,16 @@ __global__ __launch_bounds__(256) void k_df_tension_apply(const float4 *__restri
         const float ri = rest[i];
         const uint32_t nn = cnt[i] & ~kDeadBit;
         float sx = 0, sy = 0, sz = 0;
-        struct D { float4 x, n; float r; int32_t m; };
+        // same-material neighbours: the fluid ones (non-zero mass), whose
+        // rest density is rho0_f (as in k_df_vis_normal)
+        struct D { float4 x, n; };
         for_each_neighbor(
-            nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg(normal, j), ldg(rest, j), ldg(mat, j)}; },
+            nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg(normal, j)}; },
             [&](const D &d) {
-                if (d.m != mi) return;
+                if (d.x.w == 0.0f) return;
                 float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
                 float r = sqrtf(dx * dx + dy * dy + dz * dz);
-                float kk = 2.0f * ri / (ri + d.r);
+                float kk = 2.0f * ri / (ri + q.rho0_f);
                 float c1 = -q.gamma * xi.w * d.x.w * surface_tension_c(r, q.k.h, q.c_sigma, q.h6_64) / r;
                 float c2 = -q.gamma * xi.w;
                 sx += kk * (c1 * dx + c2 * (ni.x - d.n.x));
@@ -624,8 +630,8 @@ void dfsph_step(Ctx &c) {
     c.timer.end(c.stream);
     const uint32_t bv = c.block("df_vis_normal");
     c.timer.begin(c.stream, "df_vis_normal");
-    hipLaunchKernelGGL(k_df_vis_normal, dim3(div_up(n, bv)), dim3(bv), 0, c.stream, x, v, c.mat.p, c.rest_density.p,
-                       c.vis.p, u.nbr.p, u.cnt.p, n, u.stride, q, c.acc.p, c.normal.p);
+    hipLaunchKernelGGL(k_df_vis_normal, dim3(div_up(n, bv)), dim3(bv), 0, c.stream, x, v, c.mat.p, c.vis.p, u.nbr.p,
+                       u.cnt.p, n, u.stride, q, c.acc.p, c.normal.p);
     c.timer.end(c.stream);
     c.timer.begin(c.stream, "df_tension_apply");
     hipLaunchKernelGGL(k_df_tension_apply, dim3(div_up(n, bv)), dim3(bv), 0, c.stream, x, v, c.vadv.p, c.mat.p,
