@@ -314,7 +314,9 @@ vtp_status vtp_load_state(vtp_ctx *ctx, const void *host_src, size_t bytes);
  * afterwards, so the simulation is not advanced.  1 = the workgroup size of
  * each pass (64, 128 or 256; results bit-identical); 2 = also 1, 2 or 4
  * lanes per particle in the PBF passes, which changes the FP32 rounding of
- * neighbour sums.  The chosen shapes read back through vtp_get_option),
+ * neighbour sums.  The chosen shapes read back through vtp_get_option.  As
+ * after vtp_load_state, a recorded PBF rollout is dropped, the lists count
+ * as stale until the next tick, and recorded stage times are cleared),
  * "block" (threads per block for the neighbour passes: 64, 128, 192 or 256;
  * 0 = default 256),
  * "stage_timing" (1 = record hipEvents around every stage; see

@@ -102,4 +102,6 @@ def test_pbf_autotune_mode_2_picks_a_lane_count():
     print(f"mode-2 shapes at {wl.n} particles: {shapes}", flush=True)
     c.step(3)
     assert c.stats()["crash_flags"] == 0
+    with pytest.raises(vtp.VtpError, match="unknown neighbour pass"):
+        c.get_option("block.no_such_pass")
     c.close()

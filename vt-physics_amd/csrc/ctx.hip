@@ -7,6 +7,7 @@
 // slot order and `orig` maps slot -> attach id.
 #include <algorithm>
 #include <cmath>
+#include <iterator>
 #include <cstring>
 #include <numeric>
 
@@ -737,7 +738,19 @@ extern "C" vtp_status vtp_get_option(vtp_ctx *ctx, const char *key, double *valu
     else if (k == "build_lambda") *value = c.opt_build_lambda;
     else if (k == "radix_sort") *value = c.ugns.radix;
     else if (k == "neighbor_walk") *value = c.ugns.walk;
-    else if (k.rfind("block.", 0) == 0) *value = c.block(k.c_str() + 6);
+    else if (k.rfind("block.", 0) == 0) {
+        static const char *const passes[] = {"pbf_lambda", "pbf_dx", "pbf_xsph", "df_precompute", "df_delta_kappa",
+                                              "df_update_vadv", "df_vis_normal", "imm_precompute", "imm_vis_normal",
+                                              "imm_tension_phase_vel", "imm_fluxes", "immct_conformation",
+                                              "immct_stress_acc", "imm_pass"};
+        const std::string pass = k.substr(6);
+        if (std::find(std::begin(passes), std::end(passes), pass) == std::end(passes))
+            return ctx_fail(c, VTP_EINVAL, "vtp_get_option: unknown neighbour pass '" + pass + "'");
+        // the IMM passes fall back to the "imm_pass" size, the others to "block"
+        auto it = c.block_of.find(pass);
+        *value = it != c.block_of.end() ? it->second
+                 : pass.rfind("imm", 0) == 0 ? c.block("imm_pass") : c.opt_block;
+    }
     else return ctx_fail(c, VTP_EINVAL, "vtp_get_option: unknown option '" + k + "'");
     return VTP_OK;
 }
