@@ -136,7 +136,7 @@ template <uint32_t L>
 __global__ __launch_bounds__(256) void k_adj_xsph(const float4 *__restrict__ xvu, const int32_t *__restrict__ mat,
                                                   const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ cnt,
                                                   uint32_t n, uint32_t stride, PbfK q, float4 *__restrict__ xb,
-                                                  float4 *__restrict__ vb, double *kbar,
+                                                  float4 *__restrict__ vb, double *__restrict__ kpart,
                                                   const float4 *__restrict__ Xw, float4 *__restrict__ xw) {
     const uint32_t g = slot_index(), p = g / L, sub = g % L;
     const bool fluid = p < n && mat[p] == VTP_MAT_FLUID;
@@ -192,8 +192,34 @@ __global__ __launch_bounds__(256) void k_adj_xsph(const float4 *__restrict__ xvu
             if (Xw) write_xw(xw, p, Xw[p], true, a, b, q.inv_dt);
         }
     }
-    double s = wave_sum(kp);
-    if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(kbar, s);
+    // one FP64 partial of the XSPH-coefficient adjoint per workgroup, summed
+    // in a fixed order by k_adj_kbar (no shared-address atomics: a fixed
+    // result, and no serialisation of thousands of atomics on one word)
+    __shared__ double s_part[kMaxWavesPerBlock];
+    const double ws = wave_sum(kp);
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = ws;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) t += s_part[w];
+        kpart[blockIdx.x] = t;
+    }
+}
+
+// kbar += the partials of one XSPH adjoint launch, summed by 256 strided lanes
+// and a fixed tree.
+__global__ __launch_bounds__(256) void k_adj_kbar(const double *__restrict__ kpart, uint32_t nparts,
+                                                  double *__restrict__ kbar) {
+    __shared__ double s_red[256];
+    double t = 0.0;
+    for (uint32_t k = threadIdx.x; k < nparts; k += 256u) t += kpart[k];
+    s_red[threadIdx.x] = t;
+    __syncthreads();
+    for (uint32_t off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) s_red[threadIdx.x] += s_red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *kbar += s_red[0];
 }
 
 // lambda_bar_i = (1/rho0) sum_j (w_i - w_j) . gradW_ij, then the adjoints of the
@@ -425,7 +451,7 @@ struct AdjointState {
     ~AdjointState() { drop_graphs(); }
     DevBuf<float4> xb, vb, xb2, vb2;
     DevBuf<float4> xvu, xw, rec;   // gather records of the reverse passes
-    DevBuf<double> kbar;
+    DevBuf<double> kbar, kpart;   // dL/dXSPH_k and one partial per XSPH-adjoint workgroup
     DevBuf<int32_t> mat;
     bool valid = false;
 };
@@ -503,6 +529,7 @@ static void ensure_tapes(Ctx &c, AdjointState &A) {
     A.xw.alloc(2 * (size_t)n);
     A.rec.alloc(4 * (size_t)n);
     A.kbar.alloc(1);
+    A.kpart.alloc(div_up(4 * n, 256) + 1);   // workgroups of k_adj_xsph at up to 4 lanes per particle
     A.mat.alloc(n);
     A.canon[0] = A.xb.p;
     A.canon[1] = A.vb.p;
@@ -528,10 +555,11 @@ static void reverse_tick(Ctx &c, AdjointState &A, TapeBufs &t) {
     const uint32_t L = adj_lanes(c), nbl = div_up(n * L, 256);
     if (L == 4)
         hipLaunchKernelGGL(k_adj_xsph<4>, dim3(nbl), dim3(256), 0, c.stream, A.xvu.p, A.mat.p, t.nbr.p, t.cnt.p, n,
-                           u.stride, q, A.xb.p, A.vb.p, A.kbar.p, xw_of((int)K - 1), A.xw.p);
+                           u.stride, q, A.xb.p, A.vb.p, A.kpart.p, xw_of((int)K - 1), A.xw.p);
     else
         hipLaunchKernelGGL(k_adj_xsph<1>, dim3(nbl), dim3(256), 0, c.stream, A.xvu.p, A.mat.p, t.nbr.p, t.cnt.p, n,
-                           u.stride, q, A.xb.p, A.vb.p, A.kbar.p, xw_of((int)K - 1), A.xw.p);
+                           u.stride, q, A.xb.p, A.vb.p, A.kpart.p, xw_of((int)K - 1), A.xw.p);
+    hipLaunchKernelGGL(k_adj_kbar, dim3(1), dim3(256), 0, c.stream, A.kpart.p, nbl, A.kbar.p);
     c.timer.end(c.stream);
     for (int m = (int)K - 1; m >= 0; --m) {
         c.timer.begin(c.stream, "adj_iteration");
