@@ -160,6 +160,29 @@ def test_torch_autograd_on_device_tensors_matches_host_path():
     assert np.abs(out["cuda"][3]).max() > 0 and out["cuda"][4] != 0
 
 
+def test_device_read_write_round_trip_and_errors():
+    """vtp_write_device / vtp_read_device move attach-order float[n][3]
+    arrays through device memory: a written velocity field reads back
+    unchanged (device and host paths agree), and wrong sizes or fields are
+    refused with VTP_EINVAL."""
+    torch = pytest.importorskip("torch")
+    wl = scene(seed=8)
+    p, ctx = setup(wl)
+    n = wl.n
+    v = torch.randn((n, 3), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    ctx.write_device("vel", v.data_ptr())
+    back = torch.empty_like(v)
+    ctx.read_device("vel", back.data_ptr())
+    np.testing.assert_array_equal(back.cpu().numpy(), v.cpu().numpy())
+    np.testing.assert_array_equal(ctx.read("vel"), v.cpu().numpy())
+    lib = vtp.load_library()
+    assert lib.vtp_read_device(ctx.h, vtp._abi.FIELDS["pos"][0], back.data_ptr(), 12 * n - 12) == vtp._abi.VTP_EINVAL
+    assert lib.vtp_write_device(ctx.h, vtp._abi.FIELDS["density"][0], back.data_ptr(), 12 * n) == vtp._abi.VTP_EINVAL
+    assert lib.vtp_read_device(ctx.h, vtp._abi.FIELDS["pos"][0], None, 12 * n) == vtp._abi.VTP_EINVAL
+    ctx.close()
+
+
 def _cluster_ctx(k, K=1, xsph=0.0, seed=2):
     """k^3 jittered fluid lattice at spacing 0.018 (no walls) with N(0, 0.1^2)
     velocities, config-0 physics otherwise."""
