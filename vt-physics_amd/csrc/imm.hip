@@ -631,9 +631,14 @@ __global__ __launch_bounds__(256) void k_imm_fluxes(const float4 *__restrict__ x
                     const float4 dj = d.pd[k];
                     float c1 = -dt * Vj * ((fi[k] * di[k].x + fj * dj.x) * gx + (fi[k] * di[k].y + fj * dj.y) * gy +
                                            (fi[k] * di[k].z + fj * dj.z) * gz);
-                    if (c1 < 0.0f) fout[k] += c1; else fin[k] += c1;
+                    // negative terms to fout, the rest to fin, written as
+                    // selects of the addend (adding +0 leaves either sum as
+                    // it is) so that fin and fout stay in registers
+                    fout[k] += c1 < 0.0f ? c1 : 0.0f;
+                    fin[k] += c1 < 0.0f ? 0.0f : c1;
                     float c2 = dt * q.cf * (fi[k] - fj) * Vj * factor;
-                    if (c2 < 0.0f) fout[k] += c2; else fin[k] += c2;
+                    fout[k] += c2 < 0.0f ? c2 : 0.0f;
+                    fin[k] += c2 < 0.0f ? 0.0f : c2;
                 }
             });
     }

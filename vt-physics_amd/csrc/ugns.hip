@@ -16,10 +16,11 @@
 //  * because the input is the previous step's slot order rather than attach
 //    order, equal cell ids are put in attach-id order by a per-cell rank pass,
 //    which reproduces thrust's stable (cellId, attachId) order exactly;
-//  * neighbour lists hold 16-bit slot offsets from a per-particle base in an
-//    ELL layout (column t of slot i at col[t*stride + i]: lane i of a
-//    wavefront reads halfword i, fully coalesced; layout in vtp_device.h)
-//    without the self entry; the reference layout is rebuilt on request.
+//  * neighbour lists hold 16-bit slot offsets from a per-particle base,
+//    eight to a 16-byte chunk, chunk q of slot i at chunk q * stride + i
+//    (lane i of a wavefront reads chunk i: 1 KB, fully coalesced; layout in
+//    vtp_device.h), without the self entry; the reference layout is rebuilt
+//    on request.
 #include <algorithm>
 
 #include "vtp_internal.h"
@@ -58,6 +59,7 @@ constexpr uint32_t kNbrBuildThreads = 128;
 #define VTP_REF_ORDER_SHARE 8
 #endif
 constexpr uint32_t kNbrStage = VTP_NBR_STAGE;   // list entries per thread staged in LDS (8 KB per workgroup)
+static_assert(VTP_NBR_STAGE % 8 == 0 && VTP_NBR_STAGE_LAMBDA % 8 == 0, "the stage is flushed in whole 8-entry chunks");
 // The PBF build that also computes lambda needs more registers (6 waves per
 // SIMD), which leaves room for a larger stage at the same occupancy.
 constexpr uint32_t kNbrStageLambda = VTP_NBR_STAGE_LAMBDA;   // 12 KB per workgroup
@@ -540,7 +542,8 @@ __global__ __launch_bounds__(kNbrBuildThreads, VTP_BUILD_MIN_WAVES) void k_build
     auto stage_slot = [&](uint32_t c) { return ((c >> 1) * blockDim.x + threadIdx.x) * 2u + (c & 1u); };
     const uint32_t i = slot_index();
     uint32_t *const wide_flag = nbr + 3 * (size_t)stride;
-    uint16_t *const col = nbr_cols(nbr, stride);
+    uint4 *const chunks = nbr_chunks_of(nbr, stride);
+    uint16_t *const narrow = reinterpret_cast<uint16_t *>(chunks);
     if (kWide && !__builtin_amdgcn_readfirstlane(i < stride ? wide_flag[i >> 6] : 0u)) return;
     // uniform over the grid; walk = 1 / 2 forces the row / reference walk
     const bool ref_walk = walk == 2 || (walk == 0 && status[kStNbrPrev] > n / kRefOrderShare);
@@ -556,13 +559,12 @@ __global__ __launch_bounds__(kNbrBuildThreads, VTP_BUILD_MIN_WAVES) void k_build
     // (its top two bits the layer) whenever the offset fits
     auto put = [&](uint32_t c, uint32_t j, const float4 &xj, uint32_t ztag) {
         if (kWide) {
-            col[(size_t)(2 * c) * stride + i] = (uint16_t)(j & 0xFFFFu);
-            col[(size_t)(2 * c + 1) * stride + i] = (uint16_t)(j >> 16);
+            reinterpret_cast<uint32_t *>(chunks)[nbr_wide_pos(stride, i, c)] = j;
             return;
         }
         const uint16_t e = (uint16_t)(j - ztag);
         if (c < stage) s_list[stage_slot(c)] = e;
-        else col[(size_t)c * stride + i] = e;
+        else narrow[nbr_narrow_pos(stride, i, c)] = e;
         if (kLam && sums) pbf_lambda_pair<kLam == 2>(xi, xj, lo.q.k, rho, gsx, gsy, gsz, s2);
     };
     auto restart = [&]() {   // the self term first, as the lambda pass
@@ -704,10 +706,17 @@ __global__ __launch_bounds__(kNbrBuildThreads, VTP_BUILD_MIN_WAVES) void k_build
     const bool wide = __ballot(over) != 0;
     if ((threadIdx.x & 63u) == 0 && i < stride) wide_flag[i >> 6] = wide ? 1u : 0u;
     if (kStaged && !wide) {
-        // every lane reads back only its own LDS column: no barrier needed
+        // every lane reads back only its own LDS column (no barrier needed)
+        // and stores it in whole chunks; stage is a multiple of 8 whenever
+        // entries past it went to memory directly, so no chunk mixes the two
         const uint32_t cmax = min(wave_max_u32(c), stage);
-        for (uint32_t cc = 0; cc < cmax; ++cc)
-            if (cc < c) col[(size_t)cc * stride + i] = s_list[stage_slot(cc)];
+        const uint32_t *words = reinterpret_cast<const uint32_t *>(s_list);
+        for (uint32_t q = 0; 8 * q < cmax; ++q)
+            if (8 * q < c) {
+                const uint32_t w = 4 * q * blockDim.x + threadIdx.x;
+                chunks[(size_t)q * stride + i] =
+                    make_uint4(words[w], words[w + blockDim.x], words[w + 2 * blockDim.x], words[w + 3 * blockDim.x]);
+            }
     }
 }
 
@@ -853,9 +862,10 @@ void Ugns::build_neighbors(const float4 *x, hipStream_t s, StageTimer &tm, const
     if (!status) throw std::runtime_error("neighbour build without status words");
     static_rows = fluid_only == nullptr;
     tm.begin(s, "ugns_neighbors");
-    // staged entries pair up in 32-bit words: an odd stage takes a whole last word
+    // the stage is read back in whole 8-entry chunks: a stage cut by a small
+    // cap is rounded up to one
     const uint32_t staged = std::min(cap, lam ? kNbrStageLambda : kNbrStage);
-    const size_t lds = (size_t)((staged + 1) / 2) * 2 * kNbrBuildThreads * sizeof(uint16_t);
+    const size_t lds = (size_t)((staged + 7) / 8) * 8 * kNbrBuildThreads * sizeof(uint16_t);
     const dim3 grid(blocks_for(n, kNbrBuildThreads)), block(kNbrBuildThreads);
 #define VTP_BUILD(LAM, OUT)                                                                                   \
     hipLaunchKernelGGL((k_build_neighbors<true, LAM, false>), grid, block, lds, s, x, key[sorted].p, start.p,   \

@@ -206,30 +206,46 @@ __device__ __forceinline__ float ldg_w(const float4 *__restrict__ base, uint32_t
 //                                  non-empty cell row in key order)
 //   uint32_t wide[flag words]      one word per 64-slot group: 1 = that group's
 //                                  lists hold full 32-bit entries
-//   uint16_t col[2 cap][stride]    column-major (ELL) entries: entry t of slot i
-//                                  is e = col[t][i] with base[e >> 14][i] +
-//                                  (e & 0x3FFF) in a narrow group, and
-//                                  col[2t][i] | col[2t+1][i] << 16 in a wide one
+//   16-byte chunks chunk[q][stride] chunk q of slot i at chunk index
+//                                  q * stride + i: a wavefront reads 1 KB of
+//                                  consecutive chunks.  In a narrow group
+//                                  chunk q holds entries 8q .. 8q + 7 as
+//                                  16-bit e, entry = base[e >> 14][i] +
+//                                  (e & 0x3FFF); in a wide group it holds
+//                                  entries 4q .. 4q + 3 as 32-bit slots.
 // Keys are z-major, so the 9 cells of one layer span three cell rows and a
 // neighbour lies at most two rows (~2 gx cells) past its layer's base: 14
 // bits hold that unless a cell row carries more than ~5k particles, and then
-// only that particle's 64-slot group switches to 32-bit entries.  Half the
-// index bytes of a 32-bit list: a wave reads one 128-byte line per column.
-// stride is a multiple of 64; the flag words are padded so that the columns
-// start on a 256-byte boundary.
+// only that particle's 64-slot group switches to 32-bit entries.  A pass
+// fetches eight entries with one 16-byte load, so its index loads are one
+// vector-memory instruction per eight neighbours, not one per neighbour.
+// Entries past a list's length in its last chunk are stale; readers never
+// use them.  stride is a multiple of 64; the flag words are padded so that
+// the chunks start on a 256-byte boundary.
 constexpr uint32_t kNbrOffBits = 14;
 constexpr uint32_t kNbrOffMask = (1u << kNbrOffBits) - 1u;
 __host__ __device__ inline uint32_t nbr_flag_words(uint32_t stride) { return ((stride / 64u + 63u) / 64u) * 64u; }
+// chunks per slot: enough for cap 32-bit entries, rounded to whole 8-entry
+// batches (two chunks each in a wide group)
+__host__ __device__ inline uint32_t nbr_chunks(uint32_t cap) { return 2u * ((cap + 7u) / 8u); }
 __host__ __device__ inline size_t nbr_words(uint32_t cap, uint32_t stride) {
-    return 3 * (size_t)stride + nbr_flag_words(stride) + (size_t)cap * stride;   // 2 cap u16 columns = cap words
+    return 3 * (size_t)stride + nbr_flag_words(stride) + (size_t)nbr_chunks(cap) * 4u * stride;
 }
-__host__ __device__ inline uint16_t *nbr_cols(uint32_t *nbr, uint32_t stride) {
-    return reinterpret_cast<uint16_t *>(nbr + 3 * (size_t)stride + nbr_flag_words(stride));
+__host__ __device__ inline uint4 *nbr_chunks_of(uint32_t *nbr, uint32_t stride) {
+    return reinterpret_cast<uint4 *>(nbr + 3 * (size_t)stride + nbr_flag_words(stride));
 }
-__host__ __device__ inline const uint16_t *nbr_cols(const uint32_t *nbr, uint32_t stride) {
-    return reinterpret_cast<const uint16_t *>(nbr + 3 * (size_t)stride + nbr_flag_words(stride));
+__host__ __device__ inline const uint4 *nbr_chunks_of(const uint32_t *nbr, uint32_t stride) {
+    return reinterpret_cast<const uint4 *>(nbr + 3 * (size_t)stride + nbr_flag_words(stride));
 }
 __host__ __device__ inline uint32_t nbr_flag_index(uint32_t stride, uint32_t i) { return 3 * stride + (i >> 6); }
+// Element positions of entry t of slot i: a 16-bit index into the chunks for
+// a narrow entry, a 32-bit index for a wide one.
+__host__ __device__ inline size_t nbr_narrow_pos(uint32_t stride, uint32_t i, uint32_t t) {
+    return ((size_t)(t >> 3) * stride + i) * 8u + (t & 7u);
+}
+__host__ __device__ inline size_t nbr_wide_pos(uint32_t stride, uint32_t i, uint32_t t) {
+    return ((size_t)(t >> 2) * stride + i) * 4u + (t & 3u);
+}
 __device__ __forceinline__ uint32_t nbr_decode(uint32_t e, uint32_t b0, uint32_t b1, uint32_t b2) {
     const uint32_t z = e >> kNbrOffBits;
     return (z == 0 ? b0 : z == 1 ? b1 : b2) + (e & kNbrOffMask);
@@ -237,73 +253,98 @@ __device__ __forceinline__ uint32_t nbr_decode(uint32_t e, uint32_t b0, uint32_t
 // Entry t of slot i's list, decoded per lane (for kernels off the hot path).
 __device__ __forceinline__ uint32_t nbr_entry(const uint32_t *__restrict__ nbr, uint32_t stride, uint32_t i,
                                               uint32_t t) {
-    const uint16_t *col = nbr_cols(nbr, stride);
-    if (nbr[nbr_flag_index(stride, i)])
-        return (uint32_t)col[(size_t)(2 * t) * stride + i] | ((uint32_t)col[(size_t)(2 * t + 1) * stride + i] << 16);
-    return nbr_decode(col[(size_t)t * stride + i], nbr[i], nbr[stride + i], nbr[2 * (size_t)stride + i]);
+    const uint4 *ch = nbr_chunks_of(nbr, stride);
+    if (nbr[nbr_flag_index(stride, i)]) return reinterpret_cast<const uint32_t *>(ch)[nbr_wide_pos(stride, i, t)];
+    return nbr_decode(reinterpret_cast<const uint16_t *>(ch)[nbr_narrow_pos(stride, i, t)], nbr[i], nbr[stride + i],
+                      nbr[2 * (size_t)stride + i]);
 }
 
-// Column readers of one slot's list.  The column base is wave-uniform (t is
-// the same in every lane), and the address is a 32-bit byte offset from it.
+// Chunk readers of one slot's list.  `row` is the byte offset of the slot's
+// chunk 0 (i * 16); chunk q is `step` bytes further on per q.  batch<B>(t, j)
+// fills j[0 .. B) with entries t .. t + B - 1 (t a multiple of B, B = 4 or 8)
+// from one or two 16-byte loads; entry(t) reads one entry.
 struct NarrowIdx {
-    const char *col;
+    const char *ch;
     size_t step;
-    uint32_t off, b0, b1, b2;
-    __device__ __forceinline__ uint32_t operator()(uint32_t t) const {
-        return nbr_decode(*reinterpret_cast<const uint16_t *>(col + t * step + off), b0, b1, b2);
+    uint32_t row, b0, b1, b2;
+    __device__ __forceinline__ uint32_t entry(uint32_t t) const {
+        const char *p = ch + (size_t)(t >> 3) * step;
+        return nbr_decode(*reinterpret_cast<const uint16_t *>(p + (row + (t & 7u) * 2u)), b0, b1, b2);
+    }
+    template <uint32_t B>
+    __device__ __forceinline__ void batch(uint32_t t, uint32_t *j) const {
+        const char *p = ch + (size_t)(t >> 3) * step;
+        if (B == 8) {
+            const uint4 w = *reinterpret_cast<const uint4 *>(p + row);
+            const uint32_t v[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                j[2 * k] = nbr_decode(v[k] & 0xFFFFu, b0, b1, b2);
+                j[2 * k + 1] = nbr_decode(v[k] >> 16, b0, b1, b2);
+            }
+        } else {
+            const uint2 w = *reinterpret_cast<const uint2 *>(p + (row + (t & 7u) * 2u));
+            j[0] = nbr_decode(w.x & 0xFFFFu, b0, b1, b2);
+            j[1] = nbr_decode(w.x >> 16, b0, b1, b2);
+            j[2] = nbr_decode(w.y & 0xFFFFu, b0, b1, b2);
+            j[3] = nbr_decode(w.y >> 16, b0, b1, b2);
+        }
     }
 };
 struct WideIdx {
-    const char *col;
+    const char *ch;
     size_t step;
-    uint32_t off;
-    __device__ __forceinline__ uint32_t operator()(uint32_t t) const {
-        const uint32_t lo = *reinterpret_cast<const uint16_t *>(col + (2 * t) * step + off);
-        const uint32_t hi = *reinterpret_cast<const uint16_t *>(col + (2 * t + 1) * step + off);
-        return lo | (hi << 16);
+    uint32_t row;
+    __device__ __forceinline__ uint32_t entry(uint32_t t) const {
+        const char *p = ch + (size_t)(t >> 2) * step;
+        return *reinterpret_cast<const uint32_t *>(p + (row + (t & 3u) * 4u));
+    }
+    template <uint32_t B>
+    __device__ __forceinline__ void batch(uint32_t t, uint32_t *j) const {
+#pragma unroll
+        for (uint32_t h = 0; h < B / 4; ++h) {
+            const uint4 w = *reinterpret_cast<const uint4 *>(ch + (size_t)((t >> 2) + h) * step + row);
+            j[4 * h] = w.x;
+            j[4 * h + 1] = w.y;
+            j[4 * h + 2] = w.z;
+            j[4 * h + 3] = w.w;
+        }
     }
 };
 
 // Walks slot i's list in batches: the gathers of a batch are issued
-// together, and the indices of the next batch are requested before the
+// together, and the entries of the next batch are requested before the
 // current batch is used, so the index -> gather dependency of one batch
 // overlaps the arithmetic of the previous one.  A batch is 8 neighbours when
 // a neighbour's data fits in 32 bytes and 4 when it is larger (register
-// pressure).
+// pressure).  The last batch may be partial: its stale entries are replaced
+// by `self` (a valid slot) for the gathers and not used.
 template <typename Idx, typename Load, typename Use>
-__device__ __forceinline__ void walk_list(const Idx &idx, uint32_t nn, Load &&load, Use &&use) {
+__device__ __forceinline__ void walk_list(const Idx &idx, uint32_t self, uint32_t nn, Load &&load, Use &&use) {
     using D = decltype(load(0u));
-    constexpr uint32_t kNbrBatch = sizeof(D) > 32 ? VTP_NBR_BATCH / 2 : VTP_NBR_BATCH;
+    constexpr uint32_t B = sizeof(D) > 32 ? VTP_NBR_BATCH / 2 : VTP_NBR_BATCH;
+    static_assert(B == 4 || B == 8, "a batch is half or all of an 8-entry chunk");
+    if (nn == 0) return;
+    uint32_t j[B];
+    idx.template batch<B>(0, j);
     uint32_t t = 0;
-    if (nn >= kNbrBatch) {
-        uint32_t j[kNbrBatch];
+    for (; t + B <= nn; t += B) {
+        D d[B];
 #pragma unroll
-        for (uint32_t b = 0; b < kNbrBatch; ++b) j[b] = idx(b);
-        for (;;) {
-            D d[kNbrBatch];
+        for (uint32_t b = 0; b < B; ++b) d[b] = load(j[b]);
+        if (t + B < nn) idx.template batch<B>(t + B, j);
 #pragma unroll
-            for (uint32_t b = 0; b < kNbrBatch; ++b) d[b] = load(j[b]);
-            const bool more = t + 2 * kNbrBatch <= nn;
-            if (more) {
-#pragma unroll
-                for (uint32_t b = 0; b < kNbrBatch; ++b) j[b] = idx(t + kNbrBatch + b);
-            }
-#pragma unroll
-            for (uint32_t b = 0; b < kNbrBatch; ++b) use(d[b]);
-            t += kNbrBatch;
-            if (!more) break;
-        }
+        for (uint32_t b = 0; b < B; ++b) use(d[b]);
     }
-    // remainder: batches of four, then singles
-    for (; t + 4 <= nn; t += 4) {
-        const uint32_t j0 = idx(t), j1 = idx(t + 1), j2 = idx(t + 2), j3 = idx(t + 3);
-        D d0 = load(j0), d1 = load(j1), d2 = load(j2), d3 = load(j3);
-        use(d0);
-        use(d1);
-        use(d2);
-        use(d3);
+    if (t < nn) {
+        const uint32_t rem = nn - t;
+        D d[B];
+#pragma unroll
+        for (uint32_t b = 0; b < B; ++b) d[b] = load(b < rem ? j[b] : self);
+#pragma unroll
+        for (uint32_t b = 0; b < B; ++b)
+            if (b < rem) use(d[b]);
     }
-    for (; t < nn; ++t) use(load(idx(t)));
 }
 
 // The list of slot i, nn entries.  The format flag belongs to i's 64-slot
@@ -313,13 +354,13 @@ __device__ __forceinline__ void walk_list(const Idx &idx, uint32_t nn, Load &&lo
 template <typename Load, typename Use>
 __device__ __forceinline__ void for_each_neighbor(const uint32_t *__restrict__ nbr, uint32_t stride, uint32_t i,
                                                   uint32_t nn, Load &&load, Use &&use) {
-    const char *col = reinterpret_cast<const char *>(nbr_cols(nbr, stride));
-    const size_t step = (size_t)stride * sizeof(uint16_t);
-    const uint32_t off = i * (uint32_t)sizeof(uint16_t);
+    const char *ch = reinterpret_cast<const char *>(nbr_chunks_of(nbr, stride));
+    const size_t step = (size_t)stride * sizeof(uint4);
+    const uint32_t row = i * (uint32_t)sizeof(uint4);
     if (__builtin_amdgcn_readfirstlane(nbr[nbr_flag_index(stride, i)]))
-        walk_list(WideIdx{col, step, off}, nn, load, use);
+        walk_list(WideIdx{ch, step, row}, i, nn, load, use);
     else
-        walk_list(NarrowIdx{col, step, off, nbr[i], nbr[stride + i], nbr[2 * (size_t)stride + i]}, nn, load, use);
+        walk_list(NarrowIdx{ch, step, row, nbr[i], nbr[stride + i], nbr[2 * (size_t)stride + i]}, i, nn, load, use);
 }
 
 // Cooperative walk: L lanes share one particle, lane `sub` taking entries
@@ -335,14 +376,14 @@ __device__ __forceinline__ void walk_list_coop(const Idx &idx, uint32_t nn, uint
     for (; t + (B - 1) * L < nn; t += B * L) {
         uint32_t j[B];
 #pragma unroll
-        for (uint32_t b = 0; b < B; ++b) j[b] = idx(t + b * L);
+        for (uint32_t b = 0; b < B; ++b) j[b] = idx.entry(t + b * L);
         D d[B];
 #pragma unroll
         for (uint32_t b = 0; b < B; ++b) d[b] = load(j[b]);
 #pragma unroll
         for (uint32_t b = 0; b < B; ++b) use(d[b]);
     }
-    for (; t < nn; t += L) use(load(idx(t)));
+    for (; t < nn; t += L) use(load(idx.entry(t)));
 }
 
 template <uint32_t L, typename Load, typename Use>
@@ -353,13 +394,13 @@ __device__ __forceinline__ void for_each_neighbor_coop(const uint32_t *__restric
         for_each_neighbor(nbr, stride, i, nn, load, use);
         return;
     }
-    const char *col = reinterpret_cast<const char *>(nbr_cols(nbr, stride));
-    const size_t step = (size_t)stride * sizeof(uint16_t);
-    const uint32_t off = i * (uint32_t)sizeof(uint16_t);
+    const char *ch = reinterpret_cast<const char *>(nbr_chunks_of(nbr, stride));
+    const size_t step = (size_t)stride * sizeof(uint4);
+    const uint32_t row = i * (uint32_t)sizeof(uint4);
     if (__builtin_amdgcn_readfirstlane(nbr[nbr_flag_index(stride, i)]))
-        walk_list_coop<L>(WideIdx{col, step, off}, nn, sub, load, use);
+        walk_list_coop<L>(WideIdx{ch, step, row}, nn, sub, load, use);
     else
-        walk_list_coop<L>(NarrowIdx{col, step, off, nbr[i], nbr[stride + i], nbr[2 * (size_t)stride + i]}, nn, sub,
+        walk_list_coop<L>(NarrowIdx{ch, step, row, nbr[i], nbr[stride + i], nbr[2 * (size_t)stride + i]}, nn, sub,
                           load, use);
 }
 
