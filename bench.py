@@ -364,9 +364,11 @@ LEG_FIRST_KERNEL = (("dfsph_1m", "k_df_"), ("batched_64", "k_pbf_"), ("adjoint",
 
 def trace_summary(trace_csv):
     """Markdown tables from a rocprofv3 --kernel-trace CSV of a default bench
-    run: per kernel and leg, launches and microseconds per launch.  A leg
-    starts at the first dispatch of its own kernel family, in the order the
-    bench runs them (PBF 1M headline, DFSPH 1M, batched, adjoint, IMM-CT 1M)."""
+    run: per kernel and leg, launches and microseconds per launch.  Legs run
+    in the bench's order (PBF 1M headline, DFSPH 1M, batched, adjoint, IMM-CT
+    1M); a leg starts at the k_iota (the first kernel of a context's
+    initialisation) that precedes the first dispatch of its own kernel
+    family, so its contexts' first neighbour search counts to it."""
     import csv
     from collections import defaultdict
 
@@ -374,10 +376,18 @@ def trace_summary(trace_csv):
                     r["Kernel_Name"].split("(")[0].replace("void ", "").replace("vtp::", ""),
                     int(r["Grid_Size_X"]))
                    for r in csv.DictReader(open(trace_csv))))
-    legs, leg, want = [], "pbf_1m (headline)", list(LEG_FIRST_KERNEL)
-    for t0, t1, name, grid in rows:
-        if want and name.startswith(want[0][1]):
-            leg = want.pop(0)[0]
+    starts, lo = [], 0
+    for leg_name, prefix in LEG_FIRST_KERNEL:
+        first = next((k for k in range(lo, len(rows)) if rows[k][2].startswith(prefix)), None)
+        if first is None:
+            break
+        init = next((k for k in range(first, lo - 1, -1) if rows[k][2] == "k_iota"), first)
+        starts.append((init, leg_name))
+        lo = first
+    legs, leg = [], "pbf_1m (headline)"
+    for k, (t0, t1, name, grid) in enumerate(rows):
+        if starts and k == starts[0][0]:
+            leg = starts.pop(0)[1]
         legs.append((leg, name, grid, t1 - t0))
     acc = defaultdict(list)
     for leg, name, grid, d in legs:
