@@ -301,22 +301,29 @@ __global__ __launch_bounds__(256) void k_df_update_vadv(const float4 *__restrict
 // or more than `cap` iterations ran (then the crash bit is raised).  One
 // workgroup sums the per-workgroup partials of the delta/kappa pass in a
 // fixed order.
-__global__ __launch_bounds__(256) void k_df_decide(Loop lp, uint32_t nparts, float thr, uint32_t cap,
-                                                   uint32_t iter_word, uint32_t crash_bit, int which) {
+#ifndef VTP_DECIDE_THREADS
+#define VTP_DECIDE_THREADS 1024
+#endif
+// One workgroup of kDecideThreads: every thread sums a strided share of the
+// partials, each wave reduces its lanes with shuffles and one barrier joins
+// the waves' sums, so the reduction order is fixed for a given partial count.
+constexpr uint32_t kDecideThreads = VTP_DECIDE_THREADS;
+__global__ __launch_bounds__(kDecideThreads) void k_df_decide(Loop lp, uint32_t nparts, float thr, uint32_t cap,
+                                                               uint32_t iter_word, uint32_t crash_bit, int which) {
     if (lp.status[kStDone]) return;
-    __shared__ double s_red[256];
+    __shared__ double s_red[kDecideThreads / 64];
     double t = 0.0;
-    for (uint32_t k = threadIdx.x; k < nparts; k += blockDim.x) t += lp.partial[k];
-    s_red[threadIdx.x] = t;
+#pragma unroll 4
+    for (uint32_t k = threadIdx.x; k < nparts; k += kDecideThreads) t += lp.partial[k];
+    t = wave_sum(t);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = t;
     __syncthreads();
-    for (uint32_t off = blockDim.x / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) s_red[threadIdx.x] += s_red[threadIdx.x + off];
-        __syncthreads();
-    }
     if (threadIdx.x != 0) return;
+    double res = 0.0;
+#pragma unroll
+    for (uint32_t w = 0; w < kDecideThreads / 64; ++w) res += s_red[w];
     const uint32_t iter = lp.status[kStIter] + 1;
     lp.status[kStIter] = iter;
-    const double res = s_red[0];
     if (which == 0) lp.sc->div_residual = res; else lp.sc->inc_residual = res;
     if (res < (double)thr || iter > cap) {
         lp.status[kStDone] = 1;
@@ -514,8 +521,8 @@ void df_pressure_solve(Ctx &c, const float4 *x, const float4 *v, int which) {
             hipLaunchKernelGGL(k_df_update_vadv, dim3(div_up(n, bu)), dim3(bu), 0, c.stream, x, v, c.xk.p, c.g1.p,
                                c.vadv.p, c.mat.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, kap, lp);
             c.timer.end(c.stream);
-            hipLaunchKernelGGL(k_df_decide, dim3(1), dim3(256), 0, c.stream, lp, div_up(n, bd), thr, cap, word,
-                               bit, which);
+            hipLaunchKernelGGL(k_df_decide, dim3(1), dim3(kDecideThreads), 0, c.stream, lp, div_up(n, bd), thr, cap,
+                               word, bit, which);
         }
         read_status(c);
         if (c.h_status[kStDone] || launched > cap) break;
