@@ -451,6 +451,12 @@ extern "C" vtp_status vtp_read(vtp_ctx *ctx, int32_t field, void *host_dst, size
             if (!d) throw std::runtime_error("field not available for this solver");
             to_attach(c, d, 1, orig, out);
         };
+        // a table already in attach order (per-particle constants set at initialize)
+        auto attach1 = [&](const float *d) {
+            if (!d) throw std::runtime_error("field not available for this solver");
+            VTP_HIP(hipMemcpyAsync(out, d, (size_t)n * 4, hipMemcpyDeviceToHost, c.stream));
+            VTP_HIP(hipStreamSynchronize(c.stream));
+        };
         // phase-major [k*stride + slot] arrays -> [attach*P + k] (comps values each)
         auto phased = [&](const void *d, size_t elem_floats, size_t comps) {
             if (!d) throw std::runtime_error("field not available for this solver");
@@ -486,7 +492,9 @@ extern "C" vtp_status vtp_read(vtp_ctx *ctx, int32_t field, void *host_dst, size
             case VTP_FIELD_DENSITY: f1(c.dens.p); break;
             case VTP_FIELD_LAMBDA: f1(c.lam.p); break;
             case VTP_FIELD_ERROR: f1(c.err_c.p); break;
-            case VTP_FIELD_REST_DENSITY: f1(c.P ? c.rest2[0].p : c.rest_density.p); break;
+            case VTP_FIELD_REST_DENSITY:
+                if (c.P) f1(c.rest2[0].p); else attach1(c.rest_attach.p);
+                break;
             case VTP_FIELD_COMPRESSION_RATIO: f1(c.cr.p); break;
             case VTP_FIELD_DF_ALPHA: f1(c.alpha.p); break;
             case VTP_FIELD_DELTA_CR: f1(c.delta.p); break;
@@ -495,7 +503,9 @@ extern "C" vtp_status vtp_read(vtp_ctx *ctx, int32_t field, void *host_dst, size
             case VTP_FIELD_VEL_ADV: f4_xyz(c.vadv.p); break;
             case VTP_FIELD_ACC: f4_xyz(c.acc.p); break;
             case VTP_FIELD_SURFACE_NORMAL: f4_xyz(c.normal.p); break;
-            case VTP_FIELD_VIS: f1(c.p.solver == VTP_IMMCT ? c.vis_ct.p : c.vis.p); break;
+            case VTP_FIELD_VIS:
+                if (c.p.solver == VTP_IMMCT) f1(c.vis_ct.p); else if (c.P) f1(c.vis.p); else attach1(c.vis_attach.p);
+                break;
             case VTP_FIELD_VOL_FRAC: phased(c.frac[0].p, 1, 1); break;
             case VTP_FIELD_VEL_PHASE: phased(c.uphase[0].p, 4, 3); break;
             case VTP_FIELD_VEL_DRIFT: phased(c.drift.p, 4, 3); break;

@@ -95,12 +95,6 @@ __global__ void k_scatter_attach_f32(const float *__restrict__ src, const uint32
     if (i < n) dst[orig[i]] = src[i];
 }
 
-__global__ void k_gather_attach_f32(const float *__restrict__ tab, const uint32_t *__restrict__ orig, uint32_t n,
-                                    float *__restrict__ dst) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = tab[orig[i]];
-}
-
 // --------------------------------------------------------------- CFL (extension)
 __global__ void k_df_vmax(const float4 *__restrict__ v, const int32_t *__restrict__ mat, uint32_t n,
                           SolverScalars *sc) {
@@ -374,7 +368,6 @@ __global__ void k_df_apply_pressure(const float4 *__restrict__ x, float4 *__rest
 //   n_i = h sum_j m_j/rho0_j gradW_ij.
 __global__ __launch_bounds__(256) void k_df_vis_normal(const float4 *__restrict__ x, const float4 *__restrict__ v,
                                                        const int32_t *__restrict__ mat,
-                                                       const float *__restrict__ vis,
                                                        const uint32_t *__restrict__ nbr,
                                                        const uint32_t *__restrict__ cnt, uint32_t n,
                                                        uint32_t stride, DfK q, float4 *__restrict__ acc,
@@ -394,8 +387,9 @@ __global__ __launch_bounds__(256) void k_df_vis_normal(const float4 *__restrict_
     // Same-material neighbours of a fluid particle are the fluid ones, those
     // with non-zero mass (boundary m = 0), whose rest density is rho0_f and
     // whose viscosity is the solver's one fPartRestViscosity (k_df_init sets
-    // every particle's), so a neighbour is its (x, m) and (v, V) records.
-    const float visi = vis[i];
+    // every particle's, and nothing changes them), so a neighbour is its
+    // (x, m) and (v, V) records and mu_i is that viscosity too.
+    const float visi = q.vis0;
     const float vis_ij = 0.5f * (visi + visi);   // (mu_i + mu_j) / 2 with mu_j = mu_i
     struct D { float4 x, v; };
     for_each_neighbor(
@@ -427,7 +421,6 @@ __global__ __launch_bounds__(256) void k_df_vis_normal(const float4 *__restrict_
 __global__ __launch_bounds__(256) void k_df_tension_apply(const float4 *__restrict__ x, float4 *__restrict__ v,
                                                           float4 *__restrict__ vadv,
                                                           const int32_t *__restrict__ mat,
-                                                          const float *__restrict__ rest,
                                                           const float4 *__restrict__ normal,
                                                           const uint32_t *__restrict__ nbr,
                                                           const uint32_t *__restrict__ cnt, uint32_t n,
@@ -441,7 +434,7 @@ __global__ __launch_bounds__(256) void k_df_tension_apply(const float4 *__restri
     if (q.gamma != 0.0f) {
         const float4 xi = x[i];
         const float4 ni = normal[i];
-        const float ri = rest[i];
+        const float ri = q.rho0_f;   // a fluid particle's rest density (k_df_init)
         const uint32_t nn = cnt[i] & ~kDeadBit;
         float sx = 0, sy = 0, sz = 0;
         // same-material neighbours: the fluid ones (non-zero mass), whose
@@ -619,9 +612,6 @@ void dfsph_step(Ctx &c) {
     u.sort_and_rank(c.orig.p, c.stream, c.timer);
     c.timer.begin(c.stream, "df_reorder");
     common_reorder(c, c.X[0].p, c.X[1].p, nullptr, c.V[0].p, c.V[1].p, nullptr);
-    hipLaunchKernelGGL(k_gather_attach_f32, dim3(nb), dim3(256), 0, c.stream, c.rest_attach.p, c.orig.p, n,
-                       c.rest_density.p);
-    hipLaunchKernelGGL(k_gather_attach_f32, dim3(nb), dim3(256), 0, c.stream, c.vis_attach.p, c.orig.p, n, c.vis.p);
     c.timer.end(c.stream);
     u.build_neighbors(c.X[1].p, c.stream, c.timer);
     float4 *x = c.X[1].p, *v = c.V[1].p;
@@ -637,12 +627,12 @@ void dfsph_step(Ctx &c) {
     c.timer.end(c.stream);
     const uint32_t bv = c.block("df_vis_normal");
     c.timer.begin(c.stream, "df_vis_normal");
-    hipLaunchKernelGGL(k_df_vis_normal, dim3(div_up(n, bv)), dim3(bv), 0, c.stream, x, v, c.mat.p, c.vis.p, u.nbr.p,
+    hipLaunchKernelGGL(k_df_vis_normal, dim3(div_up(n, bv)), dim3(bv), 0, c.stream, x, v, c.mat.p, u.nbr.p,
                        u.cnt.p, n, u.stride, q, c.acc.p, c.normal.p);
     c.timer.end(c.stream);
     c.timer.begin(c.stream, "df_tension_apply");
     hipLaunchKernelGGL(k_df_tension_apply, dim3(div_up(n, bv)), dim3(bv), 0, c.stream, x, v, c.vadv.p, c.mat.p,
-                       c.rest_density.p, c.normal.p, u.nbr.p, u.cnt.p, n, u.stride, q, c.acc.p, c.scal.p);
+                       c.normal.p, u.nbr.p, u.cnt.p, n, u.stride, q, c.acc.p, c.scal.p);
     c.timer.end(c.stream);
     df_pressure_solve(c, x, v, 1);
     c.timer.begin(c.stream, "df_apply_pressure_update_pos");

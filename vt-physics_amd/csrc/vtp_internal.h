@@ -361,6 +361,7 @@ struct Ctx {
     DevBuf<float> frac[2], frac_in, frac_out;
     DevBuf<float4> uphase[2], aphase, drift;
     DevBuf<uint32_t> flag;
+    DevBuf<float> veff;         // IMM: volume of an unflagged fluid slot, 0 otherwise (phase fluxes)
     DevBuf<float> phase_rho, phase_vis;
     DevBuf<float4> phase_color;
     DevBuf<float4> color;
