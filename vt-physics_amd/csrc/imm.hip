@@ -283,7 +283,8 @@ __global__ void k_imm_pressure_to_phases(float4 *__restrict__ x, float4 *__restr
 template <int MP>
 __global__ __launch_bounds__(256) void k_imm_vis_normal(const float4 *__restrict__ x, const float4 *__restrict__ v,
                                                         const int32_t *__restrict__ mat,
-                                                        const float *__restrict__ rest, const float *__restrict__ cd,
+                                                        const float *__restrict__ rest, const float *__restrict__ rs,
+                                                        const float *__restrict__ cd,
                                                         const uint32_t *__restrict__ nbr,
                                                         const uint32_t *__restrict__ cnt, uint32_t n,
                                                         uint32_t stride, ImmK q, Ph ph,
@@ -306,11 +307,13 @@ __global__ __launch_bounds__(256) void k_imm_vis_normal(const float4 *__restrict
         az[k] = q.gz;
     }
     float nx = 0, ny = 0, nz = 0;
-    struct D { float4 x, v; float r; int32_t m; };
+    // a neighbour's signed rest density: > 0 exactly for the fluid ones, the
+    // same material as i
+    struct D { float4 x, v; float r; };
     for_each_neighbor(
-        nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg(v, j), ldg(rest, j), ldg(mat, j)}; },
+        nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg(v, j), ldg(rs, j)}; },
         [&](const D &d) {
-            if (d.m != mi) return;
+            if (!(d.r > 0.0f)) return;
             float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
             float r2 = dx * dx + dy * dy + dz * dz;
             float f = cubic_grad_factor(r2, q.k);
@@ -343,6 +346,7 @@ __global__ __launch_bounds__(256) void k_imm_tension_phase_vel(const float4 *__r
                                                                float4 *__restrict__ vadv,
                                                                const int32_t *__restrict__ mat,
                                                                const float *__restrict__ rest,
+                                                               const float *__restrict__ rs,
                                                                const float4 *__restrict__ normal,
                                                                const uint32_t *__restrict__ nbr,
                                                                const uint32_t *__restrict__ cnt, uint32_t n,
@@ -359,11 +363,11 @@ __global__ __launch_bounds__(256) void k_imm_tension_phase_vel(const float4 *__r
         const float4 ni = normal[i];
         const float ri = rest[i];
         const uint32_t nn = cnt[i] & ~kDeadBit;
-        struct D { float4 x, n; float r; int32_t m; };
+        struct D { float4 x, n; float r; };   // r: signed rest density (> 0: fluid)
         for_each_neighbor(
-            nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg(normal, j), ldg(rest, j), ldg(mat, j)}; },
+            nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg(normal, j), ldg(rs, j)}; },
             [&](const D &d) {
-                if (d.m != mi) return;
+                if (!(d.r > 0.0f)) return;
                 float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
                 float r = sqrtf(dx * dx + dy * dy + dz * dz);
                 float kk = 2.0f * ri / (ri + d.r);
@@ -400,7 +404,7 @@ __global__ __launch_bounds__(256) void k_imm_tension_phase_vel(const float4 *__r
 __global__ __launch_bounds__(256) void k_immct_conformation(const float4 *__restrict__ x,
                                                             const float4 *__restrict__ v,
                                                             const int32_t *__restrict__ mat,
-                                                            const float *__restrict__ rest,
+                                                            const float *__restrict__ rs,
                                                             const uint32_t *__restrict__ nbr,
                                                             const uint32_t *__restrict__ cnt, uint32_t n,
                                                             uint32_t stride, ImmK q, const float *__restrict__ fpoly,
@@ -422,11 +426,12 @@ __global__ __launch_bounds__(256) void k_immct_conformation(const float4 *__rest
     const float4 vi = v[i];
     const uint32_t nn = cnt[i] & ~kDeadBit;
     float L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    struct D { float4 x, v; float r; int32_t m; };
+    // a neighbour's signed rest density: > 0 exactly for the fluid ones
+    struct D { float4 x, v; float r; };
     for_each_neighbor(
-        nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg(v, j), ldg(rest, j), ldg(mat, j)}; },
+        nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg(v, j), ldg(rs, j)}; },
         [&](const D &d) {
-            if (d.m != VTP_MAT_FLUID) return;
+            if (!(d.r > 0.0f)) return;
             float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
             float f = cubic_grad_factor(dx * dx + dy * dy + dz * dz, q.k) * (d.x.w / d.r);
             float gx = dx * f, gy = dy * f, gz = dz * f;
@@ -480,7 +485,7 @@ __global__ __launch_bounds__(256) void k_immct_conformation(const float4 *__rest
         stress[a * st + i] = S[a];
     }
     // the polymer-stress pass gathers S / rest^2 of a neighbour as one 48-byte record
-    const float ri = rest[i];
+    const float ri = rs[i];   // i is fluid: its signed rest density is its rest density
     const float ir2 = 1.0f / (ri * ri);
     srec[3 * (size_t)i] = make_float4(S[0] * ir2, S[1] * ir2, S[2] * ir2, S[3] * ir2);
     srec[3 * (size_t)i + 1] = make_float4(S[4] * ir2, S[5] * ir2, S[6] * ir2, S[7] * ir2);
@@ -766,6 +771,18 @@ __global__ void k_gather_f32(const float *__restrict__ src, float *__restrict__ 
     uint32_t o = perm[i];
     for (uint32_t c = 0; c < comps; ++c) dst[(size_t)c * stride + i] = src[(size_t)c * stride + o];
 }
+// The rest densities into the new slot order, and the signed copy the
+// neighbour passes gather: +rest for a fluid slot, -rest for a static one
+// (rest >= 0), so one 4-byte gather gives a neighbour's rest density and
+// whether it is fluid.  mat is already in the new order.
+__global__ void k_gather_rest(const float *__restrict__ src, float *__restrict__ dst, float *__restrict__ rs,
+                              const int32_t *__restrict__ mat, const uint32_t *__restrict__ perm, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float r = src[perm[i]];
+    dst[i] = r;
+    rs[i] = mat[i] == VTP_MAT_FLUID ? r : -r;
+}
 __global__ void k_gather_f4(const float4 *__restrict__ src, float4 *__restrict__ dst,
                             const uint32_t *__restrict__ perm, uint32_t n, uint32_t stride, uint32_t comps) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -828,6 +845,7 @@ void imm_initialize(Ctx &c) {
     c.frac_out.alloc((size_t)P * st);
     c.flag.alloc(st);
     c.veff.alloc(st);
+    c.rs.alloc(st);
     c.cd.alloc(st);
     c.fpoly.alloc(st);
     c.vis_ct.alloc(st);
@@ -870,7 +888,8 @@ static void imm_step_impl(Ctx &c) {
     hipLaunchKernelGGL(k_gather_f32, dim3(nb), dim3(256), 0, c.stream, c.frac[0].p, c.frac[1].p, u.perm.p, n, st, P);
     hipLaunchKernelGGL(k_gather_f4, dim3(nb), dim3(256), 0, c.stream, c.uphase[0].p, c.uphase[1].p, u.perm.p, n, st,
                        P);
-    hipLaunchKernelGGL(k_gather_f32, dim3(nb), dim3(256), 0, c.stream, c.rest2[0].p, c.rest2[1].p, u.perm.p, n, st, 1u);
+    hipLaunchKernelGGL(k_gather_rest, dim3(nb), dim3(256), 0, c.stream, c.rest2[0].p, c.rest2[1].p, c.rs.p, c.mat.p,
+                       u.perm.p, n);
     hipLaunchKernelGGL(k_gather_f4, dim3(nb), dim3(256), 0, c.stream, c.color4[0].p, c.color4[1].p, u.perm.p, n, st,
                        1u);
     if (q.ct)
@@ -907,17 +926,17 @@ static void imm_step_impl(Ctx &c) {
                        c.mat.p, rest, c.cd.p, n, q, ph, c.scal.p, 0, nullptr);
     c.timer.end(c.stream);
     c.timer.begin(c.stream, "imm_vis_normal");
-    hipLaunchKernelGGL(k_imm_vis_normal<MP>, s_vis.first, s_vis.second, 0, c.stream, x, v, c.mat.p, rest, c.cd.p, u.nbr.p,
-                       u.cnt.p, n, st, q, ph, c.normal.p);
+    hipLaunchKernelGGL(k_imm_vis_normal<MP>, s_vis.first, s_vis.second, 0, c.stream, x, v, c.mat.p, rest, c.rs.p,
+                       c.cd.p, u.nbr.p, u.cnt.p, n, st, q, ph, c.normal.p);
     c.timer.end(c.stream);
     c.timer.begin(c.stream, "imm_tension_phase_vel");
     hipLaunchKernelGGL(k_imm_tension_phase_vel<MP>, s_ten.first, s_ten.second, 0, c.stream, x, v, c.vadv.p, c.mat.p, rest,
-                       c.normal.p, u.nbr.p, u.cnt.p, n, st, q, ph, c.scal.p);
+                       c.rs.p, c.normal.p, u.nbr.p, u.cnt.p, n, st, q, ph, c.scal.p);
     c.timer.end(c.stream);
     if (q.ct) {
         c.timer.begin(c.stream, "immct_conformation");
         const auto s_conf = shape("immct_conformation"), s_str = shape("immct_stress_acc");
-        hipLaunchKernelGGL(k_immct_conformation, s_conf.first, s_conf.second, 0, c.stream, x, v, c.mat.p, rest, u.nbr.p,
+        hipLaunchKernelGGL(k_immct_conformation, s_conf.first, s_conf.second, 0, c.stream, x, v, c.mat.p, c.rs.p, u.nbr.p,
                            u.cnt.p, n, st, q, c.fpoly.p, c.Q[0].p, c.Lgrad.p, c.stress.p, c.vis_ct.p, c.srec.p, c.scal.p);
         c.timer.end(c.stream);
         c.timer.begin(c.stream, "immct_stress_acc");
