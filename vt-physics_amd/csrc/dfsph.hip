@@ -130,7 +130,9 @@ __global__ __launch_bounds__(256) void k_df_precompute(const float4 *__restrict_
                                                        const uint32_t *__restrict__ nbr,
                                                        const uint32_t *__restrict__ cnt, uint32_t n,
                                                        uint32_t stride, DfK q, float *__restrict__ cr,
-                                                       float *__restrict__ alpha, float4 *__restrict__ g1) {
+                                                       float *__restrict__ alpha, float4 *__restrict__ g1,
+                                                       const uint32_t *__restrict__ obj_of,
+                                                       float4 *__restrict__ sconst) {
     uint32_t i = slot_index();
     if (i >= n) return;
     const float4 vi = v[i];
@@ -140,6 +142,7 @@ __global__ __launch_bounds__(256) void k_df_precompute(const float4 *__restrict_
         cr[i] = 0.0f;
         alpha[i] = 0.0f;
         g1[i] = make_float4(0, 0, 0, 0);
+        sconst[i] = make_float4(0.0f, 0.0f, vi.w, 0.0f);   // alpha = 0: not fluid
         return;
     }
     const uint32_t w = cnt[i];
@@ -166,6 +169,7 @@ __global__ __launch_bounds__(256) void k_df_precompute(const float4 *__restrict_
     float a = (a1x * a1x + a1y * a1y + a1z * a1z) / xi.w + a2;
     cr[i] = c;
     alpha[i] = fmaxf(a, 1e-6f);
+    sconst[i] = make_float4(c, fmaxf(a, 1e-6f), vi.w, __uint_as_float(obj_of[i]));
     g1[i] = make_float4(a1x, a1y, a1z, 0.0f);   // sum_j V_j gradW_ij, reused by every v* update of the tick
 }
 
@@ -184,15 +188,16 @@ __global__ void k_solve_begin(uint32_t *status, SolverScalars *sc, uint32_t nobj
 //   delta_i = max(0, cr_i - 1 + dt sum_j V_j (v*_i - v*_j) . gradW_ij),
 //   kappa_i = delta_i / alpha_i / dt^2 / V_i  (boundary kappa = 0),
 // with the per-object sum of delta added to sc->obj_sum in FP64.
+// sconst[i] = (cr, alpha, V, object id) from the precompute pass: constant
+// through the solves of a tick, one 16-byte load.  alpha is exactly 0 for a
+// static slot and at least 1e-6 (or NaN, which the residual then carries)
+// for a fluid one.
 __global__ __launch_bounds__(256) void k_df_delta_kappa(const float4 *__restrict__ x,
-                                                        const float4 *__restrict__ v,
                                                         const float4 *__restrict__ vadv,
-                                                        const int32_t *__restrict__ mat,
-                                                        const uint32_t *__restrict__ obj_of,
+                                                        const float4 *__restrict__ sconst,
                                                         const uint32_t *__restrict__ nbr,
                                                         const uint32_t *__restrict__ cnt, uint32_t n,
-                                                        uint32_t stride, Cubic k, const float *__restrict__ cr,
-                                                        const float *__restrict__ alpha, float *__restrict__ delta,
+                                                        uint32_t stride, Cubic k, float *__restrict__ delta,
                                                         float *__restrict__ kappa, float4 *__restrict__ xk, Loop lp,
                                                         int static_out) {
     if (lp.status[kStDone]) return;
@@ -200,7 +205,8 @@ __global__ __launch_bounds__(256) void k_df_delta_kappa(const float4 *__restrict
     const float dt = lp.sc->dt;
     double contrib = 0.0;
     if (i < n) {
-        if (mat[i] == VTP_MAT_FLUID) {
+        const float4 sc = sconst[i];
+        if (sc.y != 0.0f) {
             const float4 xi = x[i];
             const float4 ai = vadv[i];
             const uint32_t nn = cnt[i] & ~kDeadBit;
@@ -215,14 +221,14 @@ __global__ __launch_bounds__(256) void k_df_delta_kappa(const float4 *__restrict
                     float dot = (dx * (ai.x - e.a.x) + dy * (ai.y - e.a.y) + dz * (ai.z - e.a.z)) * f;
                     d = fmaf(dot, e.a.w, d);
                 });
-            float dl = fmaxf(cr[i] - 1.0f + d * dt, 0.0f);
+            float dl = fmaxf(sc.x - 1.0f + d * dt, 0.0f);
             float inv_dt = 1.0f / dt;
-            const float Vi = v[i].w;
-            const float ki = dl / alpha[i] * (inv_dt * inv_dt) / Vi;
+            const float Vi = sc.z;
+            const float ki = dl / sc.y * (inv_dt * inv_dt) / Vi;
             delta[i] = dl;
             kappa[i] = ki;
             xk[i] = make_float4(xi.x, xi.y, xi.z, Vi * ki);
-            contrib = (double)dl * lp.obj_w[obj_of[i]];
+            contrib = (double)dl * lp.obj_w[__float_as_uint(sc.w)];
         } else if (static_out) {
             // static particles do not move during a solve: their kappa and
             // (x, 0) records are written by its first iteration only
@@ -506,9 +512,9 @@ void df_pressure_solve(Ctx &c, const float4 *x, const float4 *v, int which) {
     for (;;) {
         for (uint32_t it = 0; it < chunk && launched <= cap; ++it, ++launched) {
             c.timer.begin(c.stream, name_dk);
-            hipLaunchKernelGGL(k_df_delta_kappa, dim3(div_up(n, bd)), dim3(bd), 0, c.stream, x, v, c.vadv.p, c.mat.p,
-                               c.obj_of.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, c.cr.p, c.alpha.p, c.delta.p,
-                               kap, c.xk.p, lp, launched == 0 ? 1 : 0);
+            hipLaunchKernelGGL(k_df_delta_kappa, dim3(div_up(n, bd)), dim3(bd), 0, c.stream, x, c.vadv.p, c.sconst.p,
+                               u.nbr.p, u.cnt.p, n, u.stride, c.kern, c.delta.p, kap, c.xk.p, lp,
+                               launched == 0 ? 1 : 0);
             c.timer.end(c.stream);
             c.timer.begin(c.stream, name_up);
             hipLaunchKernelGGL(k_df_update_vadv, dim3(div_up(n, bu)), dim3(bu), 0, c.stream, x, v, c.xk.p, c.g1.p,
@@ -535,6 +541,7 @@ void dfsph_allocate_common(Ctx &c) {
     c.vis_attach.alloc(n);
     c.cr.alloc(n);
     c.alpha.alloc(n);
+    c.sconst.alloc(n);
     c.delta.alloc(n);
     c.kappa_div.alloc(n);
     c.kappa.alloc(n);
@@ -618,7 +625,7 @@ void dfsph_step(Ctx &c) {
     const uint32_t bp = c.block("df_precompute");
     c.timer.begin(c.stream, "df_precompute");
     hipLaunchKernelGGL(k_df_precompute, dim3(div_up(n, bp)), dim3(bp), 0, c.stream, x, v, c.vadv.p, c.mat.p, u.nbr.p,
-                       u.cnt.p, n, u.stride, q, c.cr.p, c.alpha.p, c.g1.p);
+                       u.cnt.p, n, u.stride, q, c.cr.p, c.alpha.p, c.g1.p, c.obj_of.p, c.sconst.p);
     c.timer.end(c.stream);
     df_pressure_solve(c, x, v, 0);
     c.timer.begin(c.stream, "df_apply_pressure");

@@ -191,7 +191,9 @@ __global__ __launch_bounds__(256) void k_imm_precompute(const float4 *__restrict
                                                         const uint32_t *__restrict__ cnt, uint32_t n,
                                                         uint32_t stride, ImmK q, const float *__restrict__ fpoly,
                                                         float *__restrict__ cr, float *__restrict__ alpha,
-                                                        float *__restrict__ cd, float4 *__restrict__ g1) {
+                                                        float *__restrict__ cd, float4 *__restrict__ g1,
+                                                        const uint32_t *__restrict__ obj_of,
+                                                        float4 *__restrict__ sconst) {
     uint32_t i = slot_index();
     if (i >= n) return;
     const float4 vi = v[i];
@@ -201,6 +203,7 @@ __global__ __launch_bounds__(256) void k_imm_precompute(const float4 *__restrict
         cr[i] = 0.0f;
         alpha[i] = 0.0f;
         g1[i] = make_float4(0, 0, 0, 0);
+        sconst[i] = make_float4(0.0f, 0.0f, vi.w, 0.0f);   // alpha = 0: not fluid
         if (q.ct) cd[i] = q.cd0;
         return;
     }
@@ -225,7 +228,9 @@ __global__ __launch_bounds__(256) void k_imm_precompute(const float4 *__restrict
             if (d.m == mi) b = fmaf(d.V * d.fp, W, b);
         });
     cr[i] = c;
-    alpha[i] = fmaxf((a1x * a1x + a1y * a1y + a1z * a1z) / xi.w + a2, 1e-6f);
+    const float al = fmaxf((a1x * a1x + a1y * a1y + a1z * a1z) / xi.w + a2, 1e-6f);
+    alpha[i] = al;
+    sconst[i] = make_float4(c, al, vi.w, __uint_as_float(obj_of[i]));
     g1[i] = make_float4(a1x, a1y, a1z, 0.0f);   // sum_j V_j gradW_ij for the v* updates
     if (q.ct) cd[i] = q.cd0 + (1.0f - q.cd0) * b;
 }
@@ -918,7 +923,7 @@ static void imm_step_impl(Ctx &c) {
                s_ten = shape("imm_tension_phase_vel"), s_flux = shape("imm_fluxes");
     c.timer.begin(c.stream, "imm_precompute");
     hipLaunchKernelGGL(k_imm_precompute, s_pre.first, s_pre.second, 0, c.stream, x, v, c.vadv.p, c.mat.p, u.nbr.p, u.cnt.p, n,
-                       st, q, c.fpoly.p, c.cr.p, c.alpha.p, c.cd.p, c.g1.p);
+                       st, q, c.fpoly.p, c.cr.p, c.alpha.p, c.cd.p, c.g1.p, c.obj_of.p, c.sconst.p);
     c.timer.end(c.stream);
     df_pressure_solve(c, x, v, 0);
     c.timer.begin(c.stream, "imm_pressure_to_phases");
