@@ -53,15 +53,19 @@ def test_ugns_random_points_bit_exact(walk, radix):
 
 
 @pytest.mark.parametrize("walk", [1, 2])
-def test_ugns_dense_cap_truncation(walk):
+@pytest.mark.parametrize("max_nbr", [2, 5, 9, 17, 20, 33, 34, 41, 49])
+def test_ugns_dense_cap_truncation(walk, max_nbr):
     # 400 particles inside one cell: lists are truncated at maxNeighborNum in
-    # the reference traversal order.
+    # the reference traversal order.  Every list then holds exactly cap =
+    # maxNeighborNum - 1 entries, so the caps cover a list that ends inside,
+    # or exactly at the end of, an 8-entry chunk and one that runs past the
+    # build's 32-entry LDS stage into entries stored directly.
     rng = np.random.default_rng(5)
     pos = rng.uniform(0.1, 0.13, size=(400, 3)).astype(np.float32)
-    wl = scenes.Workload("dense", "PBF", scenes.pbf_config(lb=(0, 0, 0), size=(0.4, 0.4, 0.4), max_nbr=20),
+    wl = scenes.Workload("dense", "PBF", scenes.pbf_config(lb=(0, 0, 0), size=(0.4, 0.4, 0.4), max_nbr=max_nbr),
                          [scenes.Obj(vtp.MAT_FLUID, pos)])
     g, _ = _compare_ugns(wl, walk=walk)
-    assert (g["neighbor_num"] == 20).all()
+    assert (g["neighbor_num"] == max_nbr).all()
 
 
 @pytest.mark.parametrize("radix", [0, 1])

@@ -258,20 +258,23 @@ __global__ __launch_bounds__(256) void k_df_delta_kappa(const float4 *__restrict
 // evaluated as dt V_i / m_i (kappa_i g1_i + sum_j (V_j kappa_j) gradW_ij) with
 // g1_i = sum_j V_j gradW_ij from the precompute pass (positions are fixed
 // during the solve): a neighbour is one 16-byte (x_j, V_j kappa_j) gather.
-__global__ __launch_bounds__(256) void k_df_update_vadv(const float4 *__restrict__ x, const float4 *__restrict__ v,
+__global__ __launch_bounds__(256) void k_df_update_vadv(const float4 *__restrict__ x,
+                                                        const float4 *__restrict__ sconst,
                                                         const float4 *__restrict__ xk,
                                                         const float4 *__restrict__ g1,
-                                                        float4 *__restrict__ vadv, const int32_t *__restrict__ mat,
+                                                        float4 *__restrict__ vadv,
                                                         const uint32_t *__restrict__ nbr,
                                                         const uint32_t *__restrict__ cnt, uint32_t n,
                                                         uint32_t stride, Cubic k, const float *__restrict__ kappa,
                                                         Loop lp) {
     if (lp.status[kStDone]) return;
     uint32_t i = slot_index();
-    if (i >= n || mat[i] != VTP_MAT_FLUID) return;
+    if (i >= n) return;
+    const float4 sc = sconst[i];   // (cr, alpha, V, object): alpha = 0 for a static slot
+    if (sc.y == 0.0f) return;
     const float dt = lp.sc->dt;
     const float4 xi = x[i];
-    const float Vi = v[i].w;
+    const float Vi = sc.z;
     const float ki = kappa[i];
     const uint32_t nn = cnt[i] & ~kDeadBit;
     float ax = 0, ay = 0, az = 0;
@@ -485,7 +488,7 @@ static Loop make_loop(Ctx &c) {
 
 // Pressure solve with device-side termination; the host checks the flag once
 // per chunk of enqueued iterations.  which: 0 = divergence, 1 = density.
-void df_pressure_solve(Ctx &c, const float4 *x, const float4 *v, int which) {
+void df_pressure_solve(Ctx &c, const float4 *x, int which) {
     const uint32_t n = c.n;
     Ugns &u = c.ugns;
     Loop lp = make_loop(c);
@@ -517,8 +520,8 @@ void df_pressure_solve(Ctx &c, const float4 *x, const float4 *v, int which) {
                                launched == 0 ? 1 : 0);
             c.timer.end(c.stream);
             c.timer.begin(c.stream, name_up);
-            hipLaunchKernelGGL(k_df_update_vadv, dim3(div_up(n, bu)), dim3(bu), 0, c.stream, x, v, c.xk.p, c.g1.p,
-                               c.vadv.p, c.mat.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, kap, lp);
+            hipLaunchKernelGGL(k_df_update_vadv, dim3(div_up(n, bu)), dim3(bu), 0, c.stream, x, c.sconst.p, c.xk.p,
+                               c.g1.p, c.vadv.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, kap, lp);
             c.timer.end(c.stream);
             hipLaunchKernelGGL(k_df_decide, dim3(1), dim3(kDecideThreads), 0, c.stream, lp, div_up(n, bd), thr, cap,
                                word, bit, which);
@@ -627,7 +630,7 @@ void dfsph_step(Ctx &c) {
     hipLaunchKernelGGL(k_df_precompute, dim3(div_up(n, bp)), dim3(bp), 0, c.stream, x, v, c.vadv.p, c.mat.p, u.nbr.p,
                        u.cnt.p, n, u.stride, q, c.cr.p, c.alpha.p, c.g1.p, c.obj_of.p, c.sconst.p);
     c.timer.end(c.stream);
-    df_pressure_solve(c, x, v, 0);
+    df_pressure_solve(c, x, 0);
     c.timer.begin(c.stream, "df_apply_pressure");
     hipLaunchKernelGGL(k_df_apply_pressure, dim3(nb), dim3(256), 0, c.stream, x, v, c.vadv.p, c.acc.p, c.mat.p, n,
                        c.scal.p, nullptr, nullptr);
@@ -641,7 +644,7 @@ void dfsph_step(Ctx &c) {
     hipLaunchKernelGGL(k_df_tension_apply, dim3(div_up(n, bv)), dim3(bv), 0, c.stream, x, v, c.vadv.p, c.mat.p,
                        c.normal.p, u.nbr.p, u.cnt.p, n, u.stride, q, c.acc.p, c.scal.p);
     c.timer.end(c.stream);
-    df_pressure_solve(c, x, v, 1);
+    df_pressure_solve(c, x, 1);
     c.timer.begin(c.stream, "df_apply_pressure_update_pos");
     hipLaunchKernelGGL(k_df_apply_pressure, dim3(nb), dim3(256), 0, c.stream, x, v, c.vadv.p, c.acc.p, c.mat.p, n,
                        c.scal.p, c.X[0].p, c.V[0].p);

@@ -807,7 +807,7 @@ void common_reorder(Ctx &c, const float4 *xs, float4 *xd, float4 *xd2, const flo
 void launch_step_begin(Ctx &c);
 void launch_hash(Ctx &c, const float4 *x);
 void dfsph_allocate_common(Ctx &c);
-void df_pressure_solve(Ctx &c, const float4 *x, const float4 *v, int which);
+void df_pressure_solve(Ctx &c, const float4 *x, int which);
 
 static Ph phases(Ctx &c) {
     return Ph{c.frac[0].p, c.uphase[0].p, c.aphase.p, c.drift.p, c.frac_in.p, c.frac_out.p, c.ugns.stride};
@@ -925,7 +925,7 @@ static void imm_step_impl(Ctx &c) {
     hipLaunchKernelGGL(k_imm_precompute, s_pre.first, s_pre.second, 0, c.stream, x, v, c.vadv.p, c.mat.p, u.nbr.p, u.cnt.p, n,
                        st, q, c.fpoly.p, c.cr.p, c.alpha.p, c.cd.p, c.g1.p, c.obj_of.p, c.sconst.p);
     c.timer.end(c.stream);
-    df_pressure_solve(c, x, v, 0);
+    df_pressure_solve(c, x, 0);
     c.timer.begin(c.stream, "imm_pressure_to_phases");
     hipLaunchKernelGGL(k_imm_pressure_to_phases<MP>, dim3(nb), dim3(256), 0, c.stream, x, v, c.vadv.p, c.acc.p,
                        c.mat.p, rest, c.cd.p, n, q, ph, c.scal.p, 0, nullptr);
@@ -949,7 +949,7 @@ static void imm_step_impl(Ctx &c) {
                            c.cd.p, c.fpoly.p, c.srec.p, u.nbr.p, u.cnt.p, n, st, q, ph, c.scal.p);
         c.timer.end(c.stream);
     }
-    df_pressure_solve(c, x, v, 1);
+    df_pressure_solve(c, x, 1);
     c.timer.begin(c.stream, "imm_pressure_update_pos");
     hipLaunchKernelGGL(k_imm_pressure_to_phases<MP>, dim3(nb), dim3(256), 0, c.stream, x, v, c.vadv.p, c.acc.p,
                        c.mat.p, rest, c.cd.p, n, q, ph, c.scal.p, 1, c.veff.p);
