@@ -319,3 +319,74 @@ def test_adjoint_refuses_rollouts_with_capped_lists():
     gv0, _, _ = ctx.rollout_bwd(np.ones((ok.n, 3), np.float32), None)
     assert np.isfinite(gv0).all() and np.abs(gv0).max() > 0
     ctx.close()
+
+
+def test_adjoint_workload_loss_16_directions():
+    """The differentiable-rollout workload's own loss on a 16 x 16 x 16
+    lattice (4,096 fluid particles, the 64 x 32 x 32 workload's physics, box
+    and N(0, 0.1^2) initial velocities), 20 ticks in two checkpoint segments
+    of 10:  L = |com_T - (0.7, 0.2, 0.5)|^2 + mean_f (rho_f / rho0 - 1)^2,
+    com_T the mean fluid position after the rollout.  dL/dv0 within 1e-3
+    (L2, relative) and dL/dXSPH_k within 1e-3 of the FP64 adjoint; 16 seeded
+    unit directions of dL/dv0 within 1e-3 |dL/dv0| of central differences
+    (eps 1e-3) of the FP64 forward, and dL/dXSPH_k within 1e-3 of a central
+    difference in XSPH_k."""
+    steps, ckpt = 20, 10
+    target = np.array([0.7, 0.2, 0.5])
+    wl = scenes.cfg4_adjoint(seed=7, n_side=(16, 16, 16))
+    p, ctx = setup(wl)
+    n, nf = wl.n, wl.n_fluid
+    print(f"workload loss: {n} particles ({nf} fluid), {steps} ticks", flush=True)
+
+    def com_grad(x):
+        g = np.zeros((n, 3))
+        g[:nf] = 2 * (x[:nf].astype(np.float64).mean(0) - target) / nf
+        return g
+
+    ctx.rollout_fwd(steps, ckpt)
+    x = ctx.read("pos")
+    Ld, gL = ctx.density_loss()
+    Lg = float(((x[:nf].astype(np.float64).mean(0) - target) ** 2).sum() + Ld)
+    gv0, _, gk = ctx.rollout_bwd(com_grad(x) + gL, np.zeros((n, 3)))
+    ctx.close()
+    o = pyoracle.Oracle.from_workload(p, wl)
+    o.initialize()
+    o.rollout_fwd(steps)
+    xo = o.read(0)
+    Ldo, gLo = o.density_loss()
+    Lo = float(((xo[:nf].mean(0) - target) ** 2).sum() + Ldo)
+    rv0, _, rk = o.rollout_bwd(com_grad(xo) + gLo, np.zeros((n, 3)))
+    e = np.linalg.norm(gv0[:nf] - rv0[:nf]) / np.linalg.norm(rv0[:nf])
+    print(f"loss gpu {Lg:.9g} oracle {Lo:.9g}; dL/dv0 error {e:.2e}; dL/dk gpu {gk:.6g} oracle {rk:.6g}", flush=True)
+    assert abs(Lg - Lo) <= 1e-5 * abs(Lo)
+    assert e <= 1e-3, e
+    assert abs(gk - rk) <= 1e-3 * abs(rk), (gk, rk)
+    v_init = np.concatenate([ob.vel if ob.vel is not None else np.zeros_like(ob.pos) for ob in wl.objects])
+
+    def loss(vel0=None, k=None):
+        q = pyoracle.Oracle.from_workload(p, wl)
+        q.initialize()
+        if vel0 is not None:
+            q.write(1, vel0)
+        if k is not None:
+            q.set_xsph(k)
+        q.step(steps)
+        L, _ = q.density_loss()
+        return float(((q.read(0)[:nf].mean(0) - target) ** 2).sum() + L)
+
+    rng = np.random.default_rng(11)
+    eps, gn = 1e-3, np.linalg.norm(gv0[:nf])
+    worst = 0.0
+    for _ in range(16):
+        d = rng.normal(size=(n, 3))
+        d[nf:] = 0
+        d /= np.linalg.norm(d)
+        fd = (loss(v_init + eps * d) - loss(v_init - eps * d)) / (2 * eps)
+        an = float((gv0 * d).sum())
+        worst = max(worst, abs(fd - an) / gn)
+        assert abs(fd - an) <= 1e-3 * gn, (fd, an, gn)
+    k0 = float(p.xsph_k)
+    fdk = (loss(k=k0 + eps * k0) - loss(k=k0 - eps * k0)) / (2 * eps * k0)
+    print(f"16 directions: worst |fd - adjoint| / |dL/dv0| = {worst:.2e}; dL/dk central difference {fdk:.6g}",
+          flush=True)
+    assert abs(fdk - gk) <= 1e-3 * abs(fdk), (fdk, gk)
