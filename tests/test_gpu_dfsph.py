@@ -130,3 +130,32 @@ def test_dfsph_cfl_step_closed_form():
     assert st["sim_time"] == pytest.approx(float(dt), rel=1e-6)
     np.testing.assert_allclose(ctx.read("pos"), g + u * dt, atol=2e-7)
     ctx.close()
+
+
+def test_dfsph_two_fluid_objects_weight_the_residual_per_object():
+    """The solve residual is the mean over the fluid objects of each object's
+    mean delta (CommonFunc.cpp:10-21), so particles of a smaller object weigh
+    more.  Two fluid blocks of 8,000 and 1,000 particles (the small one
+    compressed by a 0.8x spacing, so the two objects' mean deltas differ)
+    plus the walls, 3 ticks: iteration counts equal the oracle's and both
+    exit residuals agree within 1e-4 every tick."""
+    s = 0.01
+    big = scenes.jitter(scenes.particle_cube(s / 2, (0.02, 0.02, 0.02), (20 * s, 20 * s, 20 * s)), s, 1)
+    small = scenes.jitter(scenes.particle_cube(0.4 * s, (1.5, 0.02, 0.02), (8 * s, 8 * s, 8 * s)), 0.8 * s, 2)
+    walls = scenes.particle_box(s / 2, (0.0, 0.0, 0.0), (2.0, 1.0, 1.0), 2)
+    assert big.shape[0] == 8000 and small.shape[0] == 1000
+    wl = scenes.Workload("two_blocks", "DFSPH", scenes.dfsph_config(),
+                         [scenes.Obj(vtp.MAT_FLUID, big), scenes.Obj(vtp.MAT_FLUID, small),
+                          scenes.Obj(vtp.MAT_BOUNDARY, walls)])
+    ctx, orc = _pair(wl)
+    for t in range(3):
+        ctx.step(1)
+        orc.step(1)
+        gs, rs = ctx.stats(), orc.stats()
+        print(f"tick {t + 1}: iterations {gs['div_iterations']}, {gs['incomp_iterations']}; residuals "
+              f"{gs['div_residual']:.6e} / {rs['div_residual']:.6e}, {gs['incomp_residual']:.6e} / "
+              f"{rs['incomp_residual']:.6e}", flush=True)
+        assert (gs["div_iterations"], gs["incomp_iterations"]) == (rs["div_iterations"], rs["incomp_iterations"])
+        assert abs(gs["div_residual"] - rs["div_residual"]) <= 1e-4 * abs(rs["div_residual"]) + 1e-12
+        assert abs(gs["incomp_residual"] - rs["incomp_residual"]) <= 1e-4 * abs(rs["incomp_residual"]) + 1e-12
+    ctx.close()
