@@ -327,10 +327,12 @@ vtp_status vtp_load_state(vtp_ctx *ctx, const void *host_src, size_t bytes);
  * 1 = always rows; 2 = always the reference's order.  The neighbour sets are
  * the reference's either way; the order inside a list, and so the FP32
  * rounding of neighbour sums, depends on the walk),
- * "static_lists" (PBF: 0 = default, a tick's search builds the lists of fluid
- * particles only, since no PBF kernel reads a static particle's list after
- * initialize; 1 = every list, as the reference, which vtp_debug_neighbors
- * needs after a tick),
+ * "static_lists" (PBF and DFSPH: 0 = default, a tick's search builds the
+ * lists of fluid particles only, since no PBF or DFSPH kernel reads a static
+ * particle's list after initialize; 1 = every list, as the reference, which
+ * vtp_debug_neighbors needs after a tick.  IMM and IMM-CT always build every
+ * list: their boundary rest density is a maximum over each wall particle's
+ * neighbours every tick),
  * "radix_sort" (0 = default, the search groups particles by cell with a
  * counting sort; 1 = a stable LSD radix sort of the cell keys.  Both give
  * the same (cellId, attach id) order and bit-identical results),

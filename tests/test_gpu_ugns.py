@@ -242,7 +242,7 @@ def test_ugns_per_tick_search_bit_exact(solver, walk, radix):
     ctx = vtp.Context(params)
     ctx.set_option("neighbor_walk", walk)
     ctx.set_option("radix_sort", radix)
-    if solver == "PBF":
+    if solver in ("PBF", "DFSPH"):
         ctx.set_option("static_lists", 1)
     for o in wl.objects:
         ctx.add_particles(o.pos, o.mat, o.vel, o.vol_frac, o.scene)

@@ -647,7 +647,7 @@ extern "C" vtp_status vtp_debug_neighbors(vtp_ctx *ctx, uint32_t *neighbors, uin
         return ctx_fail(c, VTP_ESTATE, "vtp_debug_neighbors: lists belong to a replaced state; tick first");
     if (!c.ugns.static_rows)
         return ctx_fail(c, VTP_ESTATE,
-                        "vtp_debug_neighbors: the last search built fluid lists only (PBF ticks); set option "
+                        "vtp_debug_neighbors: the last search built fluid lists only (PBF and DFSPH ticks); set option "
                         "static_lists=1 and tick for the reference's complete lists");
     return guarded(c, [&]() -> vtp_status {
         const uint32_t n = c.n, M = c.p.max_neighbor_num;

@@ -623,7 +623,9 @@ void dfsph_step(Ctx &c) {
     c.timer.begin(c.stream, "df_reorder");
     common_reorder(c, c.X[0].p, c.X[1].p, nullptr, c.V[0].p, c.V[1].p, nullptr);
     c.timer.end(c.stream);
-    u.build_neighbors(c.X[1].p, c.stream, c.timer);
+    // no tick kernel reads a static particle's list (k_df_init, at
+    // initialize, is the only one): fluid lists only, unless asked for all
+    u.build_neighbors(c.X[1].p, c.stream, c.timer, c.opt_static_lists ? nullptr : c.mat.p);
     float4 *x = c.X[1].p, *v = c.V[1].p;
     const uint32_t bp = c.block("df_precompute");
     c.timer.begin(c.stream, "df_precompute");
