@@ -19,6 +19,11 @@
 #ifndef VTP_PASS_MIN_WAVES
 #define VTP_PASS_MIN_WAVES 1
 #endif
+//   VTP_NBR_LDS_DECODE 1: a lane's three layer bases sit in LDS, so decoding a
+//                      16-bit entry is one LDS read instead of two selects
+#ifndef VTP_NBR_LDS_DECODE
+#define VTP_NBR_LDS_DECODE 1
+#endif
 
 namespace vtp {
 
@@ -291,6 +296,45 @@ struct NarrowIdx {
         }
     }
 };
+// Workgroup size up to which the neighbour passes keep their layer bases in
+// LDS (3 KB per workgroup); every tuned pass shape is 64, 128 or 256.
+constexpr uint32_t kNbrLdsLanes = 256;
+// The same reader with the lane's layer bases in LDS, word z * kNbrLdsLanes
+// + threadIdx.x for layer z (consecutive lanes, consecutive banks; the fixed
+// row length makes the address one shift-add).  Each lane writes and reads
+// only its own words, so no barrier is needed.
+struct NarrowIdxLds {
+    const char *ch;
+    size_t step;
+    uint32_t row;
+    const uint32_t *lds;   // this lane's word of layer 0
+    __device__ __forceinline__ uint32_t dec(uint32_t e) const {
+        return lds[(e >> kNbrOffBits) * kNbrLdsLanes] + (e & kNbrOffMask);
+    }
+    __device__ __forceinline__ uint32_t entry(uint32_t t) const {
+        const char *p = ch + (size_t)(t >> 3) * step;
+        return dec(*reinterpret_cast<const uint16_t *>(p + (row + (t & 7u) * 2u)));
+    }
+    template <uint32_t B>
+    __device__ __forceinline__ void batch(uint32_t t, uint32_t *j) const {
+        const char *p = ch + (size_t)(t >> 3) * step;
+        if (B == 8) {
+            const uint4 w = *reinterpret_cast<const uint4 *>(p + row);
+            const uint32_t v[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                j[2 * k] = dec(v[k] & 0xFFFFu);
+                j[2 * k + 1] = dec(v[k] >> 16);
+            }
+        } else {
+            const uint2 w = *reinterpret_cast<const uint2 *>(p + (row + (t & 7u) * 2u));
+            j[0] = dec(w.x & 0xFFFFu);
+            j[1] = dec(w.x >> 16);
+            j[2] = dec(w.y & 0xFFFFu);
+            j[3] = dec(w.y >> 16);
+        }
+    }
+};
 struct WideIdx {
     const char *ch;
     size_t step;
@@ -360,6 +404,16 @@ __device__ __forceinline__ void for_each_neighbor(const uint32_t *__restrict__ n
     if (__builtin_amdgcn_readfirstlane(nbr[nbr_flag_index(stride, i)]))
         walk_list(WideIdx{ch, step, row}, i, nn, load, use);
     else
+#if VTP_NBR_LDS_DECODE
+    if (blockDim.x <= kNbrLdsLanes) {
+        __shared__ uint32_t s_bases[3 * kNbrLdsLanes];
+        uint32_t *mine = s_bases + threadIdx.x;
+        mine[0] = nbr[i];
+        mine[kNbrLdsLanes] = nbr[stride + i];
+        mine[2 * kNbrLdsLanes] = nbr[2 * (size_t)stride + i];
+        walk_list(NarrowIdxLds{ch, step, row, mine}, i, nn, load, use);
+    } else
+#endif
         walk_list(NarrowIdx{ch, step, row, nbr[i], nbr[stride + i], nbr[2 * (size_t)stride + i]}, i, nn, load, use);
 }
 
