@@ -313,7 +313,7 @@ vtp_status vtp_load_state(vtp_ctx *ctx, const void *host_src, size_t bytes);
  * the fastest; each candidate runs from a state image that is loaded back
  * afterwards, so the simulation is not advanced.  1 = the workgroup size of
  * each pass (64, 128 or 256; results bit-identical); 2 = also 1, 2 or 4
- * lanes per particle in the PBF passes, which changes the FP32 rounding of
+ * lanes per particle ("lanes"), which changes the FP32 rounding of
  * neighbour sums.  The chosen shapes read back through vtp_get_option.  As
  * after vtp_load_state, a recorded PBF rollout is dropped, the lists count
  * as stale until the next tick, and recorded stage times are cleared),
@@ -321,7 +321,8 @@ vtp_status vtp_load_state(vtp_ctx *ctx, const void *host_src, size_t bytes);
  * 0 = default 256),
  * "stage_timing" (1 = record hipEvents around every stage; see
  * vtp_stage_times), "solver_chunk" (iterations launched per host check),
- * "lanes" (1, 2 or 4 lanes per particle in the PBF neighbour passes; default 1),
+ * "lanes" (1, 2 or 4 lanes per particle in the PBF neighbour passes and in
+ * the DFSPH / IMM / IMM-CT pressure-solve passes; default 1),
  * "neighbor_walk" (0 = adaptive, default: the list build walks cell rows and
  * switches to the reference's cell order once many lists reach the cap;
  * 1 = always rows; 2 = always the reference's order.  The neighbour sets are

@@ -695,7 +695,7 @@ extern "C" vtp_status vtp_set_option(vtp_ctx *ctx, const char *key, double value
             pbf_capture_reset(c);
         } else if (k == "autotune") {
             if (value != 0.0 && value != 1.0 && value != 2.0)
-                throw std::runtime_error("autotune must be 0, 1 (workgroup sizes) or 2 (also PBF lanes)");
+                throw std::runtime_error("autotune must be 0, 1 (workgroup sizes) or 2 (also lanes per particle)");
             c.opt_autotune = (int)value;
             if (c.opt_autotune && c.initialized) autotune(c, c.opt_autotune);
             pbf_capture_reset(c);

@@ -191,7 +191,8 @@ __global__ void k_solve_begin(uint32_t *status, SolverScalars *sc, uint32_t nobj
 // sconst[i] = (cr, alpha, V, object id) from the precompute pass: constant
 // through the solves of a tick, one 16-byte load.  alpha is exactly 0 for a
 // static slot and at least 1e-6 (or NaN, which the residual then carries)
-// for a fluid one.
+// for a fluid one.  L lanes per particle share its list (L = 1: one lane).
+template <uint32_t L>
 __global__ __launch_bounds__(256) void k_df_delta_kappa(const float4 *__restrict__ x,
                                                         const float4 *__restrict__ vadv,
                                                         const float4 *__restrict__ sconst,
@@ -201,7 +202,7 @@ __global__ __launch_bounds__(256) void k_df_delta_kappa(const float4 *__restrict
                                                         float *__restrict__ kappa, float4 *__restrict__ xk, Loop lp,
                                                         int static_out) {
     if (lp.status[kStDone]) return;
-    const uint32_t i = slot_index();
+    const uint32_t g = slot_index(), i = g / L, sub = g % L;
     const float dt = lp.sc->dt;
     double contrib = 0.0;
     if (i < n) {
@@ -212,8 +213,8 @@ __global__ __launch_bounds__(256) void k_df_delta_kappa(const float4 *__restrict
             const uint32_t nn = cnt[i] & ~kDeadBit;
             float d = 0.0f;
             struct D { float4 x, a; };
-            for_each_neighbor(
-                nbr, stride, i, nn,
+            for_each_neighbor_coop<L>(
+                nbr, stride, i, nn, sub,
                 [&](uint32_t j) { return D{ldg(x, j), ldg(vadv, j)}; },   // vadv.w = V_j
                 [&](const D &e) {
                     float dx = xi.x - e.x.x, dy = xi.y - e.x.y, dz = xi.z - e.x.z;
@@ -221,15 +222,18 @@ __global__ __launch_bounds__(256) void k_df_delta_kappa(const float4 *__restrict
                     float dot = (dx * (ai.x - e.a.x) + dy * (ai.y - e.a.y) + dz * (ai.z - e.a.z)) * f;
                     d = fmaf(dot, e.a.w, d);
                 });
-            float dl = fmaxf(sc.x - 1.0f + d * dt, 0.0f);
-            float inv_dt = 1.0f / dt;
-            const float Vi = sc.z;
-            const float ki = dl / sc.y * (inv_dt * inv_dt) / Vi;
-            delta[i] = dl;
-            kappa[i] = ki;
-            xk[i] = make_float4(xi.x, xi.y, xi.z, Vi * ki);
-            contrib = (double)dl * lp.obj_w[__float_as_uint(sc.w)];
-        } else if (static_out) {
+            d = lane_sum<L>(d);
+            if (sub == 0) {
+                float dl = fmaxf(sc.x - 1.0f + d * dt, 0.0f);
+                float inv_dt = 1.0f / dt;
+                const float Vi = sc.z;
+                const float ki = dl / sc.y * (inv_dt * inv_dt) / Vi;
+                delta[i] = dl;
+                kappa[i] = ki;
+                xk[i] = make_float4(xi.x, xi.y, xi.z, Vi * ki);
+                contrib = (double)dl * lp.obj_w[__float_as_uint(sc.w)];
+            }
+        } else if (static_out && sub == 0) {
             // static particles do not move during a solve: their kappa and
             // (x, 0) records are written by its first iteration only
             kappa[i] = 0.0f;
@@ -258,6 +262,7 @@ __global__ __launch_bounds__(256) void k_df_delta_kappa(const float4 *__restrict
 // evaluated as dt V_i / m_i (kappa_i g1_i + sum_j (V_j kappa_j) gradW_ij) with
 // g1_i = sum_j V_j gradW_ij from the precompute pass (positions are fixed
 // during the solve): a neighbour is one 16-byte (x_j, V_j kappa_j) gather.
+template <uint32_t L>
 __global__ __launch_bounds__(256) void k_df_update_vadv(const float4 *__restrict__ x,
                                                         const float4 *__restrict__ sconst,
                                                         const float4 *__restrict__ xk,
@@ -268,7 +273,7 @@ __global__ __launch_bounds__(256) void k_df_update_vadv(const float4 *__restrict
                                                         uint32_t stride, Cubic k, const float *__restrict__ kappa,
                                                         Loop lp) {
     if (lp.status[kStDone]) return;
-    uint32_t i = slot_index();
+    const uint32_t g = slot_index(), i = g / L, sub = g % L;
     if (i >= n) return;
     const float4 sc = sconst[i];   // (cr, alpha, V, object): alpha = 0 for a static slot
     if (sc.y == 0.0f) return;
@@ -278,8 +283,8 @@ __global__ __launch_bounds__(256) void k_df_update_vadv(const float4 *__restrict
     const float ki = kappa[i];
     const uint32_t nn = cnt[i] & ~kDeadBit;
     float ax = 0, ay = 0, az = 0;
-    for_each_neighbor(
-        nbr, stride, i, nn, [&](uint32_t j) { return ldg(xk, j); },   // (x_j, V_j kappa_j)
+    for_each_neighbor_coop<L>(
+        nbr, stride, i, nn, sub, [&](uint32_t j) { return ldg(xk, j); },   // (x_j, V_j kappa_j)
         [&](const float4 &e) {
             float dx = xi.x - e.x, dy = xi.y - e.y, dz = xi.z - e.z;
             float s = cubic_grad_factor(dx * dx + dy * dy + dz * dz, k) * e.w;
@@ -287,10 +292,14 @@ __global__ __launch_bounds__(256) void k_df_update_vadv(const float4 *__restrict
             ay = fmaf(dy, s, ay);
             az = fmaf(dz, s, az);
         });
-    const float4 g = g1[i];
-    ax = fmaf(ki, g.x, ax);
-    ay = fmaf(ki, g.y, ay);
-    az = fmaf(ki, g.z, az);
+    ax = lane_sum<L>(ax);
+    ay = lane_sum<L>(ay);
+    az = lane_sum<L>(az);
+    if (sub != 0) return;
+    const float4 gi = g1[i];
+    ax = fmaf(ki, gi.x, ax);
+    ay = fmaf(ki, gi.y, ay);
+    az = fmaf(ki, gi.z, az);
     float c = dt * Vi / xi.w;
     float4 a = vadv[i];
     a.x -= c * ax;
@@ -499,6 +508,7 @@ void df_pressure_solve(Ctx &c, const float4 *x, int which) {
     const char *name_dk = which == 0 ? "df_div_delta_kappa" : "df_inc_delta_kappa";
     const char *name_up = which == 0 ? "df_div_update_vadv" : "df_inc_update_vadv";
     const uint32_t bd = c.block("df_delta_kappa"), bu = c.block("df_update_vadv");
+    const uint32_t L = c.opt_lanes;   // lanes per particle in both solve passes
     // kappa_div and kappa_incomp are separate arrays, as in the reference
     // (DFSPHCudaImpl.cu:182-196 and 355-367), so both stay readable
     float *kap = which == 0 ? c.kappa_div.p : c.kappa.p;
@@ -515,16 +525,30 @@ void df_pressure_solve(Ctx &c, const float4 *x, int which) {
     for (;;) {
         for (uint32_t it = 0; it < chunk && launched <= cap; ++it, ++launched) {
             c.timer.begin(c.stream, name_dk);
-            hipLaunchKernelGGL(k_df_delta_kappa, dim3(div_up(n, bd)), dim3(bd), 0, c.stream, x, c.vadv.p, c.sconst.p,
-                               u.nbr.p, u.cnt.p, n, u.stride, c.kern, c.delta.p, kap, c.xk.p, lp,
-                               launched == 0 ? 1 : 0);
+            const int so = launched == 0 ? 1 : 0;
+            if (L == 4)
+                hipLaunchKernelGGL(k_df_delta_kappa<4>, dim3(div_up(4 * n, bd)), dim3(bd), 0, c.stream, x, c.vadv.p,
+                                   c.sconst.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, c.delta.p, kap, c.xk.p, lp, so);
+            else if (L == 2)
+                hipLaunchKernelGGL(k_df_delta_kappa<2>, dim3(div_up(2 * n, bd)), dim3(bd), 0, c.stream, x, c.vadv.p,
+                                   c.sconst.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, c.delta.p, kap, c.xk.p, lp, so);
+            else
+                hipLaunchKernelGGL(k_df_delta_kappa<1>, dim3(div_up(n, bd)), dim3(bd), 0, c.stream, x, c.vadv.p,
+                                   c.sconst.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, c.delta.p, kap, c.xk.p, lp, so);
             c.timer.end(c.stream);
             c.timer.begin(c.stream, name_up);
-            hipLaunchKernelGGL(k_df_update_vadv, dim3(div_up(n, bu)), dim3(bu), 0, c.stream, x, c.sconst.p, c.xk.p,
-                               c.g1.p, c.vadv.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, kap, lp);
+            if (L == 4)
+                hipLaunchKernelGGL(k_df_update_vadv<4>, dim3(div_up(4 * n, bu)), dim3(bu), 0, c.stream, x, c.sconst.p,
+                                   c.xk.p, c.g1.p, c.vadv.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, kap, lp);
+            else if (L == 2)
+                hipLaunchKernelGGL(k_df_update_vadv<2>, dim3(div_up(2 * n, bu)), dim3(bu), 0, c.stream, x, c.sconst.p,
+                                   c.xk.p, c.g1.p, c.vadv.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, kap, lp);
+            else
+                hipLaunchKernelGGL(k_df_update_vadv<1>, dim3(div_up(n, bu)), dim3(bu), 0, c.stream, x, c.sconst.p,
+                                   c.xk.p, c.g1.p, c.vadv.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, kap, lp);
             c.timer.end(c.stream);
-            hipLaunchKernelGGL(k_df_decide, dim3(1), dim3(kDecideThreads), 0, c.stream, lp, div_up(n, bd), thr, cap,
-                               word, bit, which);
+            hipLaunchKernelGGL(k_df_decide, dim3(1), dim3(kDecideThreads), 0, c.stream, lp, div_up(L * n, bd), thr,
+                               cap, word, bit, which);
         }
         read_status(c);
         if (c.h_status[kStDone] || launched > cap) break;
@@ -580,7 +604,7 @@ void dfsph_allocate_common(Ctx &c) {
         if (of[o] && oc[o]) ow[o] = 1.0 / ((double)oc[o] * (double)nfo);
     c.obj_w.alloc(no);
     VTP_HIP(hipMemcpyAsync(c.obj_w.p, ow.data(), no * sizeof(double), hipMemcpyHostToDevice, c.stream));
-    c.partial.alloc(div_up(n, 64));   // smallest workgroup of the delta/kappa pass
+    c.partial.alloc(div_up(4 * n, 64));   // 4 lanes per particle in the smallest workgroup of the delta/kappa pass
     SolverScalars s{};
     s.dt = c.p.time_step;
     VTP_HIP(hipMemcpyAsync(c.scal.p, &s, sizeof s, hipMemcpyHostToDevice, c.stream));

@@ -12,7 +12,9 @@
 // A particle's sums do not depend on it, so results stay bit-identical.
 // Mode 2: also 1, 2 or 4 lanes per particle in the PBF passes (with more
 // than one lane the first constraint iteration is not folded into the
-// neighbour build), chosen by the summed time of the build and the passes.
+// neighbour build) or in the pressure-solve passes of DFSPH, IMM and IMM-CT
+// (delta/kappa and the v* update), chosen by the summed time of the build
+// and the passes.
 // More lanes change the order of a particle's neighbour sums, so results
 // then agree with mode 1 within FP32 rounding, not bit for bit.
 #include "vtp_internal.h"
@@ -72,7 +74,7 @@ void autotune(Ctx &c, int mode) {
     c.timer.reset();
     const uint32_t cand_block[] = {64, 128, 256};   // the passes are built for at most 256 threads
     std::vector<uint32_t> cand_lanes{1};
-    if (mode == 2 && c.p.solver == VTP_PBF) cand_lanes = {1, 2, 4};
+    if (mode == 2) cand_lanes = {1, 2, 4};
     // per lane count: the best block of each pass and the summed time
     double best_total = -1;
     uint32_t best_lanes = c.opt_lanes;

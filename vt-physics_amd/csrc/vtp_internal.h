@@ -310,7 +310,7 @@ struct Ctx {
     int opt_autotune = 0;     // 0 off, 1 workgroup sizes, 2 also PBF lanes per particle
     uint32_t opt_block = 256;
     uint32_t opt_chunk = 0;   // 0 = adaptive
-    uint32_t opt_lanes = 1;   // lanes per particle in the PBF neighbour passes (1, 2 or 4)
+    uint32_t opt_lanes = 1;   // lanes per particle in the PBF passes and the pressure solves (1, 2 or 4)
     bool opt_static_lists = false;   // PBF ticks also build the (unread) lists of static particles
     bool opt_full_flux_retry = false;  // IMM: every negative-fraction retry recomputes all fluxes
     // PBF: the list build also computes the first lambda pass of a tick.  Off
