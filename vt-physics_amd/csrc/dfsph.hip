@@ -170,7 +170,9 @@ __global__ __launch_bounds__(256) void k_df_precompute(const float4 *__restrict_
     cr[i] = c;
     alpha[i] = fmaxf(a, 1e-6f);
     sconst[i] = make_float4(c, fmaxf(a, 1e-6f), vi.w, __uint_as_float(obj_of[i]));
-    g1[i] = make_float4(a1x, a1y, a1z, 0.0f);   // sum_j V_j gradW_ij, reused by every v* update of the tick
+    // sum_j V_j gradW_ij, reused by every v* update of the tick, and V_i / m_i
+    // (w != 0 marks a fluid slot for that pass)
+    g1[i] = make_float4(a1x, a1y, a1z, vi.w / xi.w);
 }
 
 // ------------------------------------------------------------------ pressure solve
@@ -263,9 +265,7 @@ __global__ __launch_bounds__(256) void k_df_delta_kappa(const float4 *__restrict
 // g1_i = sum_j V_j gradW_ij from the precompute pass (positions are fixed
 // during the solve): a neighbour is one 16-byte (x_j, V_j kappa_j) gather.
 template <uint32_t L>
-__global__ __launch_bounds__(256) void k_df_update_vadv(const float4 *__restrict__ x,
-                                                        const float4 *__restrict__ sconst,
-                                                        const float4 *__restrict__ xk,
+__global__ __launch_bounds__(256) void k_df_update_vadv(const float4 *__restrict__ xk,
                                                         const float4 *__restrict__ g1,
                                                         float4 *__restrict__ vadv,
                                                         const uint32_t *__restrict__ nbr,
@@ -275,11 +275,12 @@ __global__ __launch_bounds__(256) void k_df_update_vadv(const float4 *__restrict
     if (lp.status[kStDone]) return;
     const uint32_t g = slot_index(), i = g / L, sub = g % L;
     if (i >= n) return;
-    const float4 sc = sconst[i];   // (cr, alpha, V, object): alpha = 0 for a static slot
-    if (sc.y == 0.0f) return;
+    // g1[i] = (sum_j V_j gradW_ij, V_i / m_i), w = 0 for a static slot; the
+    // particle's own (x, V kappa) record, written by the delta pass, gives x_i
+    const float4 gi = g1[i];
+    if (gi.w == 0.0f) return;
     const float dt = lp.sc->dt;
-    const float4 xi = x[i];
-    const float Vi = sc.z;
+    const float4 xi = xk[i];
     const float ki = kappa[i];
     const uint32_t nn = cnt[i] & ~kDeadBit;
     float ax = 0, ay = 0, az = 0;
@@ -296,11 +297,10 @@ __global__ __launch_bounds__(256) void k_df_update_vadv(const float4 *__restrict
     ay = lane_sum<L>(ay);
     az = lane_sum<L>(az);
     if (sub != 0) return;
-    const float4 gi = g1[i];
     ax = fmaf(ki, gi.x, ax);
     ay = fmaf(ki, gi.y, ay);
     az = fmaf(ki, gi.z, az);
-    float c = dt * Vi / xi.w;
+    float c = dt * gi.w;
     float4 a = vadv[i];
     a.x -= c * ax;
     a.y -= c * ay;
@@ -538,14 +538,14 @@ void df_pressure_solve(Ctx &c, const float4 *x, int which) {
             c.timer.end(c.stream);
             c.timer.begin(c.stream, name_up);
             if (L == 4)
-                hipLaunchKernelGGL(k_df_update_vadv<4>, dim3(div_up(4 * n, bu)), dim3(bu), 0, c.stream, x, c.sconst.p,
-                                   c.xk.p, c.g1.p, c.vadv.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, kap, lp);
+                hipLaunchKernelGGL(k_df_update_vadv<4>, dim3(div_up(4 * n, bu)), dim3(bu), 0, c.stream, c.xk.p, c.g1.p,
+                                   c.vadv.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, kap, lp);
             else if (L == 2)
-                hipLaunchKernelGGL(k_df_update_vadv<2>, dim3(div_up(2 * n, bu)), dim3(bu), 0, c.stream, x, c.sconst.p,
-                                   c.xk.p, c.g1.p, c.vadv.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, kap, lp);
+                hipLaunchKernelGGL(k_df_update_vadv<2>, dim3(div_up(2 * n, bu)), dim3(bu), 0, c.stream, c.xk.p, c.g1.p,
+                                   c.vadv.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, kap, lp);
             else
-                hipLaunchKernelGGL(k_df_update_vadv<1>, dim3(div_up(n, bu)), dim3(bu), 0, c.stream, x, c.sconst.p,
-                                   c.xk.p, c.g1.p, c.vadv.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, kap, lp);
+                hipLaunchKernelGGL(k_df_update_vadv<1>, dim3(div_up(n, bu)), dim3(bu), 0, c.stream, c.xk.p, c.g1.p,
+                                   c.vadv.p, u.nbr.p, u.cnt.p, n, u.stride, c.kern, kap, lp);
             c.timer.end(c.stream);
             hipLaunchKernelGGL(k_df_decide, dim3(1), dim3(kDecideThreads), 0, c.stream, lp, div_up(L * n, bd), thr,
                                cap, word, bit, which);

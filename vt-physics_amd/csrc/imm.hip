@@ -231,7 +231,8 @@ __global__ __launch_bounds__(256) void k_imm_precompute(const float4 *__restrict
     const float al = fmaxf((a1x * a1x + a1y * a1y + a1z * a1z) / xi.w + a2, 1e-6f);
     alpha[i] = al;
     sconst[i] = make_float4(c, al, vi.w, __uint_as_float(obj_of[i]));
-    g1[i] = make_float4(a1x, a1y, a1z, 0.0f);   // sum_j V_j gradW_ij for the v* updates
+    // sum_j V_j gradW_ij for the v* updates, and V_i / m_i (w != 0: fluid)
+    g1[i] = make_float4(a1x, a1y, a1z, vi.w / xi.w);
     if (q.ct) cd[i] = q.cd0 + (1.0f - q.cd0) * b;
 }
 
