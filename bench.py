@@ -312,13 +312,23 @@ def cpu_model():
 STAGE_KERNEL = {"pbf_lambda": "k_pbf_lambda", "pbf_dx": "k_pbf_dx", "pbf_xsph": "k_pbf_xsph",
                 "ugns_neighbors": "k_build_neighbors"}
 TRAFFIC_JSON = ROOT / "profiles" / "pbf_dam_break_1m_traffic.json"
+# per workload: the stages whose kernels are summarised, the output file
+TRAFFIC_WORKLOADS = {
+    "pbf": ("pbf_dam_break_1m", STAGE_KERNEL, TRAFFIC_JSON),
+    "dfsph": ("dfsph_dam_break_1m",
+              {"df_precompute": "k_df_precompute", "df_delta_kappa": "k_df_delta_kappa",
+               "df_update_vadv": "k_df_update_vadv", "df_vis_normal": "k_df_vis_normal",
+               "ugns_neighbors": "k_build_neighbors"},
+              ROOT / "profiles" / "dfsph_dam_break_1m_traffic.json"),
+}
 
 
-def traffic_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, out=TRAFFIC_JSON):
+def traffic_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, workload="pbf"):
     """HBM bytes per launch of the neighbour-pass kernels from two rocprofv3
     --pmc runs (FETCH_SIZE, WRITE_SIZE, KiB per dispatch; the median over a
     kernel's launches), each scaled by what the same counter reads for a
     plain device copy of known size."""
+    wl_name, stage_kernel, out = TRAFFIC_WORKLOADS[workload]
     import csv
     from collections import defaultdict
 
@@ -348,12 +358,12 @@ def traffic_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, out=TR
     copy_kib = (1 << 30) / 1024   # calibrate_copy: 1 GiB read and 1 GiB written per dispatch
     fetch_scale, write_scale = copy_kib / cf, copy_kib / cw
     kernels = {}
-    for stage, kname in STAGE_KERNEL.items():
+    for stage, kname in stage_kernel.items():
         fk, wk = per_tick(fetch_csv, "FETCH_SIZE", kname), per_tick(write_csv, "WRITE_SIZE", kname)
         if fk is not None and wk is not None:
             rb, wb = fk * 1024 * fetch_scale, wk * 1024 * write_scale
             kernels[stage] = {"read_bytes": round(rb), "write_bytes": round(wb), "bytes": round(rb + wb)}
-    doc = {"workload": "pbf_dam_break_1m", "fetch_scale": round(fetch_scale, 4), "write_scale": round(write_scale, 4),
+    doc = {"workload": wl_name, "fetch_scale": round(fetch_scale, 4), "write_scale": round(write_scale, 4),
            "kernels": kernels}
     Path(out).write_text(json.dumps(doc, indent=1) + "\n")
     print(json.dumps(doc))
@@ -434,12 +444,17 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 1 GiB device copies (counter calibration for the traffic figure)")
     ap.add_argument("--traffic-summary", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
-                    help="turn rocprofv3 --pmc counter files into " + str(TRAFFIC_JSON.relative_to(ROOT)))
+                    help="turn rocprofv3 --pmc counter files into " + str(TRAFFIC_JSON.relative_to(ROOT)) +
+                    " (or the --traffic-workload's file)")
+    ap.add_argument("--traffic-workload", choices=sorted(TRAFFIC_WORKLOADS), default="pbf",
+                    help="which run the counter files come from: the headline (pbf) or --only-dfsph (dfsph)")
+    ap.add_argument("--only-dfsph", action="store_true",
+                    help="run only the DFSPH 1M leg and print its JSON (for profiling that leg)")
     ap.add_argument("--trace-summary", metavar="KERNEL_TRACE_CSV",
                     help="print per-leg kernel tables from a rocprofv3 --kernel-trace CSV of a bench run")
     args = ap.parse_args()
     if args.traffic_summary:
-        traffic_summary(*args.traffic_summary)
+        traffic_summary(*args.traffic_summary, workload=args.traffic_workload)
         return
     if args.trace_summary:
         trace_summary(args.trace_summary)
@@ -457,6 +472,9 @@ def main():
     torch.cuda.set_device(0)
     if args.calibrate_copy:
         calibrate_copy(torch)
+        return
+    if args.only_dfsph:
+        print(json.dumps({"dfsph_1m": bench_dfsph(torch, vtp, scenes, args.steps, args.warmup)}))
         return
     wl = scenes.pbf_1m(seed=0)
     ctx, params = build_ctx(vtp, wl, graph=True)
