@@ -245,7 +245,7 @@ __global__ void k_imm_pressure_to_phases(float4 *__restrict__ x, float4 *__restr
                                          float4 *__restrict__ vadv, float4 *__restrict__ acc,
                                          const int32_t *__restrict__ mat, const float *__restrict__ rest,
                                          const float *__restrict__ cd, uint32_t n, ImmK q, Ph ph,
-                                         const SolverScalars *sc, int update_pos, float *__restrict__ veff) {
+                                         const SolverScalars *sc, int update_pos, float4 *__restrict__ xveff) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float dt = sc->dt, inv_dt = 1.0f / dt;
@@ -254,7 +254,10 @@ __global__ void k_imm_pressure_to_phases(float4 *__restrict__ x, float4 *__restr
     float4 ac = f4((a.x - vi.x) * inv_dt, (a.y - vi.y) * inv_dt, (a.z - vi.z) * inv_dt);
     acc[i] = ac;
     if (mat[i] != VTP_MAT_FLUID) {
-        if (veff) veff[i] = 0.0f;
+        if (xveff) {
+            const float4 xi = x[i];
+            xveff[i] = make_float4(xi.x, xi.y, xi.z, 0.0f);
+        }
         return;
     }
     const float ci = cd[i], ri = rest[i];
@@ -271,13 +274,14 @@ __global__ void k_imm_pressure_to_phases(float4 *__restrict__ x, float4 *__restr
     }
     vi = mix_velocity<MP>(ph, i, q.P, vi, vadv);
     v[i] = vi;
-    if (veff) veff[i] = vi.w;   // V_i > 0: no flag is set before the phase transfer
     if (update_pos) {
         float4 xi = x[i];
         xi.x += vi.x * dt;
         xi.y += vi.y * dt;
         xi.z += vi.z * dt;
         x[i] = xi;
+        // the phase fluxes' record: V_i > 0, no flag is set before the transfer
+        if (xveff) xveff[i] = make_float4(xi.x, xi.y, xi.z, vi.w);
     }
 }
 
@@ -569,7 +573,7 @@ __global__ __launch_bounds__(256) void k_immct_stress_acc(const float4 *__restri
 // update_phase_change_from_drift + ..._from_diffuse (IMMCudaImpl.cu:532-605) into
 // fresh in/out accumulators; flagged particles (as i or j) are skipped.
 template <int MP>
-__global__ __launch_bounds__(256) void k_imm_fluxes(const float4 *__restrict__ x, const float *__restrict__ veff,
+__global__ __launch_bounds__(256) void k_imm_fluxes(const float4 *__restrict__ xveff,
                                                     const int32_t *__restrict__ mat,
                                                     const uint32_t *__restrict__ flag,
                                                     const uint32_t *__restrict__ nbr,
@@ -613,27 +617,27 @@ __global__ __launch_bounds__(256) void k_imm_fluxes(const float4 *__restrict__ x
     }
     if (!flag[i]) {
         const float dt = sc->dt;
-        const float4 xi = x[i];
+        const float4 xi = xveff[i];
         const uint32_t nn = cnt[i] & ~kDeadBit;
-        // a neighbour: x, its flux volume and per phase (drift, fraction);
-        // mix_velocity keeps the fraction in the drift's w.  veff is V for an
-        // unflagged fluid slot and 0 for a static or flagged one, the
-        // neighbours this pass skips (i is fluid, so same material = fluid).
-        struct D { float4 x; float V; float4 pd[MP]; };
+        // a neighbour: its (x, flux volume) record and per phase (drift,
+        // fraction); mix_velocity keeps the fraction in the drift's w.  The
+        // flux volume is V for an unflagged fluid slot and 0 for a static or
+        // flagged one, the neighbours this pass skips (i is fluid, so same
+        // material = fluid).
+        struct D { float4 x; float4 pd[MP]; };
         for_each_neighbor(
             nbr, stride, i, nn,
             [&](uint32_t j) {
                 D d;
-                d.x = ldg(x, j);
-                d.V = ldg(veff, j);
+                d.x = ldg(xveff, j);
 #pragma unroll
                 for (int k = 0; k < MP; ++k)
                     d.pd[k] = k < q.P ? ldg(ph.drift + (size_t)k * ph.stride, j) : make_float4(0, 0, 0, 0);
                 return d;
             },
             [&](const D &d) {
-                if (d.V == 0.0f) return;
-                const float Vj = d.V;
+                if (d.x.w == 0.0f) return;
+                const float Vj = d.x.w;
                 float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
                 float r2 = dx * dx + dy * dy + dz * dz;
                 float f = cubic_grad_factor(r2, q.k);
@@ -678,7 +682,7 @@ __global__ void k_phase_begin(uint32_t *status) {
 // decide kernel, race-free).
 template <int MP>
 __global__ void k_imm_check_negative(const int32_t *__restrict__ mat, uint32_t *__restrict__ flag,
-                                     float *__restrict__ veff, uint32_t n, ImmK q, Ph ph, uint32_t *status) {
+                                     float4 *__restrict__ xveff, uint32_t n, ImmK q, Ph ph, uint32_t *status) {
     if (status[kStDone]) return;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool neg = false;
@@ -691,7 +695,7 @@ __global__ void k_imm_check_negative(const int32_t *__restrict__ mat, uint32_t *
         }
         if (neg) {
             flag[i] = status[kStIter];   // nonzero: the round that flagged it
-            veff[i] = 0.0f;              // and no longer a flux partner
+            xveff[i].w = 0.0f;           // and no longer a flux partner
         }
     }
     uint64_t b = __ballot(neg);
@@ -850,7 +854,7 @@ void imm_initialize(Ctx &c) {
     c.frac_in.alloc((size_t)P * st);
     c.frac_out.alloc((size_t)P * st);
     c.flag.alloc(st);
-    c.veff.alloc(st);
+    c.xveff.alloc(st);
     c.rs.alloc(st);
     c.cd.alloc(st);
     c.fpoly.alloc(st);
@@ -953,13 +957,13 @@ static void imm_step_impl(Ctx &c) {
     df_pressure_solve(c, x, 1);
     c.timer.begin(c.stream, "imm_pressure_update_pos");
     hipLaunchKernelGGL(k_imm_pressure_to_phases<MP>, dim3(nb), dim3(256), 0, c.stream, x, v, c.vadv.p, c.acc.p,
-                       c.mat.p, rest, c.cd.p, n, q, ph, c.scal.p, 1, c.veff.p);
+                       c.mat.p, rest, c.cd.p, n, q, ph, c.scal.p, 1, c.xveff.p);
     c.timer.end(c.stream);
     // ---- phase transfer with the negative-fraction retry loop
     const uint32_t cap = c.p.max_solver_iterations;
     hipLaunchKernelGGL(k_phase_begin, dim3(1), dim3(64), 0, c.stream, c.status.p);
     c.timer.begin(c.stream, "imm_fluxes");
-    hipLaunchKernelGGL(k_imm_fluxes<MP>, s_flux.first, s_flux.second, 0, c.stream, x, c.veff.p, c.mat.p, c.flag.p,
+    hipLaunchKernelGGL(k_imm_fluxes<MP>, s_flux.first, s_flux.second, 0, c.stream, c.xveff.p, c.mat.p, c.flag.p,
                        u.nbr.p, u.cnt.p, n, st, q, ph, c.status.p, c.scal.p, 0);
     c.timer.end(c.stream);
     // Rounds are enqueued in chunks as in the pressure solves: the previous
@@ -975,11 +979,11 @@ static void imm_step_impl(Ctx &c) {
         for (uint32_t it = 0; it < chunk && launched <= cap; ++it, ++launched) {
             c.timer.begin(c.stream, "imm_check_negative");
             hipLaunchKernelGGL(k_imm_check_negative<MP>, dim3(nb), dim3(256), 0, c.stream, c.mat.p, c.flag.p,
-                               c.veff.p, n, q, ph, c.status.p);
+                               c.xveff.p, n, q, ph, c.status.p);
             hipLaunchKernelGGL(k_imm_phase_decide, dim3(1), dim3(64), 0, c.stream, c.status.p, cap);
             c.timer.end(c.stream);
             c.timer.begin(c.stream, "imm_fluxes");
-            hipLaunchKernelGGL(k_imm_fluxes<MP>, s_flux.first, s_flux.second, 0, c.stream, x, c.veff.p, c.mat.p,
+            hipLaunchKernelGGL(k_imm_fluxes<MP>, s_flux.first, s_flux.second, 0, c.stream, c.xveff.p, c.mat.p,
                                c.flag.p, u.nbr.p, u.cnt.p, n, st, q, ph, c.status.p, c.scal.p,
                                c.opt_full_flux_retry ? 0 : 1);
             c.timer.end(c.stream);

@@ -362,7 +362,7 @@ struct Ctx {
     DevBuf<float4> uphase[2], aphase, drift;
     DevBuf<uint32_t> flag;
     DevBuf<float4> sconst;      // DFSPH/IMM solves: (cr, alpha, V, object id bits) per slot, alpha = 0 static
-    DevBuf<float> veff;         // IMM: volume of an unflagged fluid slot, 0 otherwise (phase fluxes)
+    DevBuf<float4> xveff;       // IMM: (x, V) of an unflagged fluid slot, (x, 0) otherwise (phase fluxes)
     DevBuf<float> rs;           // IMM: rest density of a fluid slot, its negation for a static one
     DevBuf<float> phase_rho, phase_vis;
     DevBuf<float4> phase_color;
