@@ -323,12 +323,14 @@ TRAFFIC_WORKLOADS = {
 }
 
 
-def traffic_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, workload="pbf"):
+def traffic_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, workload="pbf", out=None):
     """HBM bytes per launch of the neighbour-pass kernels from two rocprofv3
     --pmc runs (FETCH_SIZE, WRITE_SIZE, KiB per dispatch; the median over a
     kernel's launches), each scaled by what the same counter reads for a
-    plain device copy of known size."""
-    wl_name, stage_kernel, out = TRAFFIC_WORKLOADS[workload]
+    plain device copy of known size.  Written to the workload's file under
+    profiles/ unless `out` names another."""
+    wl_name, stage_kernel, default_out = TRAFFIC_WORKLOADS[workload]
+    out = default_out if out is None else out
     import csv
     from collections import defaultdict
 
