@@ -181,6 +181,35 @@ __global__ void k_imm_boundary_rest(float4 *__restrict__ x, float4 *__restrict__
     }
 }
 
+// The same maximum once per tick, after the phase update (volume_pass 0 of
+// the kernel above): a static slot's rest density becomes the maximum of its
+// own part, which the phase update wrote, and its fluid neighbours' rest
+// densities, and its mass rest * V.  The fluid rest densities come from the
+// signed copy rs (> 0 exactly at fluid slots; the phase update refreshed
+// them), one 4-byte gather per neighbour in batches.  rest is updated in
+// place without a race: the pass reads rest only at its own static slot and
+// rs only at fluid slots, and writes rest only at static slots.
+__global__ __launch_bounds__(256) void k_imm_boundary_rest_tick(float4 *__restrict__ x,
+                                                                const float4 *__restrict__ v,
+                                                                const float *__restrict__ rs,
+                                                                float *__restrict__ rest,
+                                                                const int32_t *__restrict__ mat,
+                                                                const uint32_t *__restrict__ nbr,
+                                                                const uint32_t *__restrict__ cnt, uint32_t n,
+                                                                uint32_t stride) {
+    const uint32_t i = slot_index();
+    if (i >= n) return;
+    if (mat[i] == VTP_MAT_FLUID) return;
+    float r = rest[i];
+    const uint32_t nn = cnt[i] & ~kDeadBit;
+    for_each_neighbor(nbr, stride, i, nn, [&](uint32_t j) { return ldg(rs, j); },
+                      [&](float d) {
+                          if (d > 0.0f) r = fmaxf(r, d);   // fluid neighbours only
+                      });
+    rest[i] = r;
+    x[i].w = r * v[i].w;
+}
+
 // ------------------------------------------------------------------ precompute
 // compute_compression_ratio + compute_df_beta (IMMCudaImpl.cu:120-175), v* = v,
 // and for IMM-CT update_Cd_cuda (IMMCTCudaImpl.cu:782-804):
@@ -725,8 +754,8 @@ __global__ void k_imm_phase_decide(uint32_t *status, uint32_t cap) {
 template <int MP>
 __global__ void k_imm_phase_update(float4 *__restrict__ x, float4 *__restrict__ v, float4 *__restrict__ vadv,
                                    const int32_t *__restrict__ mat, uint32_t *__restrict__ flag,
-                                   float *__restrict__ rest, float4 *__restrict__ color, uint32_t n, ImmK q,
-                                   Ph ph) {
+                                   float *__restrict__ rest, float *__restrict__ rs, float4 *__restrict__ color,
+                                   uint32_t n, ImmK q, Ph ph) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (mat[i] != VTP_MAT_FLUID) {
@@ -768,37 +797,43 @@ __global__ void k_imm_phase_update(float4 *__restrict__ x, float4 *__restrict__ 
         cz += a[k] * q.col[k][2];
     }
     rest[i] = r;
+    rs[i] = r;   // fluid: the signed copy is the rest density itself
     x[i].w = r * vi.w;
     v[i] = mix_velocity<MP>(ph, i, q.P, vi, vadv);
     color[i] = f4(cx, cy, cz);
 }
 
-// gathers of the carried state into the new slot order
-__global__ void k_gather_f32(const float *__restrict__ src, float *__restrict__ dst, const uint32_t *__restrict__ perm,
-                             uint32_t n, uint32_t stride, uint32_t comps) {
+// The carried per-slot state into the new slot order, one pass and one read
+// of the permutation: phase fractions and phase velocities, the rest density
+// and its signed copy rs that the neighbour passes gather (+rest for a fluid
+// slot, -rest for a static one, rest >= 0, so one 4-byte gather gives a
+// neighbour's rest density and whether it is fluid), the colour and, for
+// IMM-CT (Q_src non-null), Q - I.  mat is already in the new order.
+template <int MP>
+__global__ void k_imm_gather_state(const uint32_t *__restrict__ perm, const int32_t *__restrict__ mat, uint32_t n,
+                                   uint32_t stride, uint32_t P, const float *__restrict__ frac_src,
+                                   float *__restrict__ frac_dst, const float4 *__restrict__ u_src,
+                                   float4 *__restrict__ u_dst, const float *__restrict__ rest_src,
+                                   float *__restrict__ rest_dst, float *__restrict__ rs,
+                                   const float4 *__restrict__ col_src, float4 *__restrict__ col_dst,
+                                   const float *__restrict__ Q_src, float *__restrict__ Q_dst) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint32_t o = perm[i];
-    for (uint32_t c = 0; c < comps; ++c) dst[(size_t)c * stride + i] = src[(size_t)c * stride + o];
-}
-// The rest densities into the new slot order, and the signed copy the
-// neighbour passes gather: +rest for a fluid slot, -rest for a static one
-// (rest >= 0), so one 4-byte gather gives a neighbour's rest density and
-// whether it is fluid.  mat is already in the new order.
-__global__ void k_gather_rest(const float *__restrict__ src, float *__restrict__ dst, float *__restrict__ rs,
-                              const int32_t *__restrict__ mat, const uint32_t *__restrict__ perm, uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float r = src[perm[i]];
-    dst[i] = r;
+    const uint32_t o = perm[i];
+#pragma unroll
+    for (int k = 0; k < MP; ++k) {
+        if (k >= (int)P) break;
+        frac_dst[(size_t)k * stride + i] = frac_src[(size_t)k * stride + o];
+        u_dst[(size_t)k * stride + i] = u_src[(size_t)k * stride + o];
+    }
+    const float r = rest_src[o];
+    rest_dst[i] = r;
     rs[i] = mat[i] == VTP_MAT_FLUID ? r : -r;
-}
-__global__ void k_gather_f4(const float4 *__restrict__ src, float4 *__restrict__ dst,
-                            const uint32_t *__restrict__ perm, uint32_t n, uint32_t stride, uint32_t comps) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t o = perm[i];
-    for (uint32_t c = 0; c < comps; ++c) dst[(size_t)c * stride + i] = src[(size_t)c * stride + o];
+    col_dst[i] = col_src[o];
+    if (Q_src) {
+#pragma unroll
+        for (int a = 0; a < 9; ++a) Q_dst[(size_t)a * stride + i] = Q_src[(size_t)a * stride + o];
+    }
 }
 __global__ void k_extract_phase(const float *__restrict__ frac, float *__restrict__ out, uint32_t n, uint32_t stride,
                                 int k) {
@@ -895,15 +930,9 @@ static void imm_step_impl(Ctx &c) {
     u.sort_and_rank(c.orig.p, c.stream, c.timer);
     c.timer.begin(c.stream, "imm_reorder");
     common_reorder(c, c.X[0].p, c.X[1].p, nullptr, c.V[0].p, c.V[1].p, nullptr);
-    hipLaunchKernelGGL(k_gather_f32, dim3(nb), dim3(256), 0, c.stream, c.frac[0].p, c.frac[1].p, u.perm.p, n, st, P);
-    hipLaunchKernelGGL(k_gather_f4, dim3(nb), dim3(256), 0, c.stream, c.uphase[0].p, c.uphase[1].p, u.perm.p, n, st,
-                       P);
-    hipLaunchKernelGGL(k_gather_rest, dim3(nb), dim3(256), 0, c.stream, c.rest2[0].p, c.rest2[1].p, c.rs.p, c.mat.p,
-                       u.perm.p, n);
-    hipLaunchKernelGGL(k_gather_f4, dim3(nb), dim3(256), 0, c.stream, c.color4[0].p, c.color4[1].p, u.perm.p, n, st,
-                       1u);
-    if (q.ct)
-        hipLaunchKernelGGL(k_gather_f32, dim3(nb), dim3(256), 0, c.stream, c.Q[0].p, c.Q[1].p, u.perm.p, n, st, 9u);
+    hipLaunchKernelGGL(k_imm_gather_state<MP>, dim3(nb), dim3(256), 0, c.stream, u.perm.p, c.mat.p, n, st, P,
+                       c.frac[0].p, c.frac[1].p, c.uphase[0].p, c.uphase[1].p, c.rest2[0].p, c.rest2[1].p, c.rs.p,
+                       c.color4[0].p, c.color4[1].p, q.ct ? c.Q[0].p : nullptr, c.Q[1].p);
     c.timer.end(c.stream);
     std::swap(c.X[0].p, c.X[1].p);
     std::swap(c.V[0].p, c.V[1].p);
@@ -997,11 +1026,9 @@ static void imm_step_impl(Ctx &c) {
     c.iter_hint_phase = rounds;
     c.timer.begin(c.stream, "imm_phase_update");
     hipLaunchKernelGGL(k_imm_phase_update<MP>, dim3(nb), dim3(256), 0, c.stream, x, v, c.vadv.p, c.mat.p, c.flag.p,
-                       rest, c.color4[0].p, n, q, ph);
-    // rest2[1] holds the previous slot order's rest densities, no longer read
-    VTP_HIP(hipMemcpyAsync(c.rest2[1].p, rest, n * 4, hipMemcpyDeviceToDevice, c.stream));
-    hipLaunchKernelGGL(k_imm_boundary_rest, dim3(nb), dim3(256), 0, c.stream, x, v, c.rest2[1].p, rest, c.mat.p,
-                       u.nbr.p, u.cnt.p, n, st, c.kern, 0);
+                       rest, c.rs.p, c.color4[0].p, n, q, ph);
+    hipLaunchKernelGGL(k_imm_boundary_rest_tick, dim3(nb), dim3(256), 0, c.stream, x, v, c.rs.p, rest, c.mat.p,
+                       u.nbr.p, u.cnt.p, n, st);
     c.timer.end(c.stream);
     VTP_HIP(hipGetLastError());
 }
