@@ -395,6 +395,7 @@ __global__ __launch_bounds__(256) void k_df_vis_normal(const float4 *__restrict_
     const int32_t mi = mat[i];
     if (mi != VTP_MAT_FLUID) {
         acc[i] = make_float4(0, 0, 0, 0);
+        normal[i] = make_float4(0, 0, 0, 0);   // slots move between ticks: clear what a fluid particle left
         return;
     }
     const float4 xi = x[i];

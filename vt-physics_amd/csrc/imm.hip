@@ -218,7 +218,7 @@ __global__ __launch_bounds__(256) void k_imm_precompute(const float4 *__restrict
                                                         float4 *__restrict__ vadv, const int32_t *__restrict__ mat,
                                                         const uint32_t *__restrict__ nbr,
                                                         const uint32_t *__restrict__ cnt, uint32_t n,
-                                                        uint32_t stride, ImmK q, const float *__restrict__ fpoly,
+                                                        uint32_t stride, ImmK q, const float4 *__restrict__ aux,
                                                         float *__restrict__ cr, float *__restrict__ alpha,
                                                         float *__restrict__ cd, float4 *__restrict__ g1,
                                                         const uint32_t *__restrict__ obj_of,
@@ -241,20 +241,23 @@ __global__ __launch_bounds__(256) void k_imm_precompute(const float4 *__restrict
     const uint32_t nn = w & ~kDeadBit;
     float c = (w & kDeadBit) ? 0.0f : vi.w * q.k.w0;
     float a1x = 0, a1y = 0, a1z = 0, a2 = 1e-6f, b = 0.0f;
-    struct D { float4 x; float V, fp; int32_t m; };
+    // a neighbour: (x_j, m_j) and (V_j, polymer fraction, material)
+    struct D { float4 x, a; };
     for_each_neighbor(
-        nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg_w(v, j), q.ct ? ldg(fpoly, j) : 0.0f, ldg(mat, j)}; },
+        nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg(aux, j)}; },
         [&](const D &d) {
             float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
             float W, f;
             cubic_w_grad(dx * dx + dy * dy + dz * dz, q.k, W, f);
-            c = fmaf(d.V, W, c);
+            const float Vj = d.a.x;
+            const int32_t mj = __float_as_int(d.a.z);
+            c = fmaf(Vj, W, c);
             float gx = dx * f, gy = dy * f, gz = dz * f;
-            a1x = fmaf(d.V, gx, a1x);
-            a1y = fmaf(d.V, gy, a1y);
-            a1z = fmaf(d.V, gz, a1z);
-            if (d.m == VTP_MAT_FLUID) a2 += (gx * gx + gy * gy + gz * gz) * d.V * d.V / d.x.w;
-            if (d.m == mi) b = fmaf(d.V * d.fp, W, b);
+            a1x = fmaf(Vj, gx, a1x);
+            a1y = fmaf(Vj, gy, a1y);
+            a1z = fmaf(Vj, gz, a1z);
+            if (mj == VTP_MAT_FLUID) a2 += (gx * gx + gy * gy + gz * gz) * Vj * Vj / d.x.w;
+            if (mj == mi) b = fmaf(Vj * d.a.y, W, b);
         });
     cr[i] = c;
     const float al = fmaxf((a1x * a1x + a1y * a1y + a1z * a1z) / xi.w + a2, 1e-6f);
@@ -322,7 +325,8 @@ __global__ void k_imm_pressure_to_phases(float4 *__restrict__ x, float4 *__restr
 template <int MP>
 __global__ __launch_bounds__(256) void k_imm_vis_normal(const float4 *__restrict__ x, const float4 *__restrict__ v,
                                                         const int32_t *__restrict__ mat,
-                                                        const float *__restrict__ rest, const float *__restrict__ rs,
+                                                        const float *__restrict__ rest,
+                                                        const float4 *__restrict__ xmr,
                                                         const float *__restrict__ cd,
                                                         const uint32_t *__restrict__ nbr,
                                                         const uint32_t *__restrict__ cnt, uint32_t n,
@@ -331,7 +335,11 @@ __global__ __launch_bounds__(256) void k_imm_vis_normal(const float4 *__restrict
     uint32_t i = slot_index();
     if (i >= n) return;
     const int32_t mi = mat[i];
-    if (mi != VTP_MAT_FLUID) return;
+    if (mi != VTP_MAT_FLUID) {
+        // slots move between ticks: clear what a fluid particle left (w = 0: not fluid)
+        normal[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
     const float4 xi = x[i];
     const float4 vi = v[i];
     const float ci = cd[i];
@@ -346,13 +354,13 @@ __global__ __launch_bounds__(256) void k_imm_vis_normal(const float4 *__restrict
         az[k] = q.gz;
     }
     float nx = 0, ny = 0, nz = 0;
-    // a neighbour's signed rest density: > 0 exactly for the fluid ones, the
-    // same material as i
-    struct D { float4 x, v; float r; };
+    // a neighbour: (x_j, m_j / signed rest density), > 0 exactly for the
+    // fluid ones, the same material as i; and v_j
+    struct D { float4 x, v; };
     for_each_neighbor(
-        nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg(v, j), ldg(rs, j)}; },
+        nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(xmr, j), ldg(v, j)}; },
         [&](const D &d) {
-            if (!(d.r > 0.0f)) return;
+            if (!(d.x.w > 0.0f)) return;
             float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
             float r2 = dx * dx + dy * dy + dz * dz;
             float f = cubic_grad_factor(r2, q.k);
@@ -367,7 +375,7 @@ __global__ __launch_bounds__(256) void k_imm_vis_normal(const float4 *__restrict
                 ay[k] = fmaf(dy, s, ay[k]);
                 az[k] = fmaf(dz, s, az[k]);
             }
-            float sn = q.k.h * d.x.w / d.r * f;
+            float sn = q.k.h * d.x.w * f;
             nx = fmaf(dx, sn, nx);
             ny = fmaf(dy, sn, ny);
             nz = fmaf(dz, sn, nz);
@@ -375,7 +383,8 @@ __global__ __launch_bounds__(256) void k_imm_vis_normal(const float4 *__restrict
 #pragma unroll
     for (int k = 0; k < MP; ++k)
         if (k < q.P) ph.a[(size_t)k * ph.stride + i] = f4(ax[k], ay[k], az[k]);
-    normal[i] = f4(nx, ny, nz);
+    // w: the rest density (> 0), which the surface-tension pass gathers with the normal
+    normal[i] = make_float4(nx, ny, nz, rest[i]);
 }
 
 // add_phase_acc_surface_tension (IMMCudaImpl.cu:348-381) + phase_acc_2_phase_vel
@@ -385,7 +394,6 @@ __global__ __launch_bounds__(256) void k_imm_tension_phase_vel(const float4 *__r
                                                                float4 *__restrict__ vadv,
                                                                const int32_t *__restrict__ mat,
                                                                const float *__restrict__ rest,
-                                                               const float *__restrict__ rs,
                                                                const float4 *__restrict__ normal,
                                                                const uint32_t *__restrict__ nbr,
                                                                const uint32_t *__restrict__ cnt, uint32_t n,
@@ -402,14 +410,15 @@ __global__ __launch_bounds__(256) void k_imm_tension_phase_vel(const float4 *__r
         const float4 ni = normal[i];
         const float ri = rest[i];
         const uint32_t nn = cnt[i] & ~kDeadBit;
-        struct D { float4 x, n; float r; };   // r: signed rest density (> 0: fluid)
+        // (x_j, m_j) and (n_j, rest density of a fluid slot, 0 for a static one)
+        struct D { float4 x, n; };
         for_each_neighbor(
-            nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg(normal, j), ldg(rs, j)}; },
+            nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg(normal, j)}; },
             [&](const D &d) {
-                if (!(d.r > 0.0f)) return;
+                if (!(d.n.w > 0.0f)) return;
                 float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
                 float r = sqrtf(dx * dx + dy * dy + dz * dz);
-                float kk = 2.0f * ri / (ri + d.r);
+                float kk = 2.0f * ri / (ri + d.n.w);
                 float c1 = -q.gamma * xi.w * d.x.w * surface_tension_c(r, q.k.h, q.c_sigma, q.h6_64) / r;
                 float c2 = -q.gamma * xi.w;
                 sx += kk * (c1 * dx + c2 * (ni.x - d.n.x));
@@ -444,6 +453,7 @@ __global__ __launch_bounds__(256) void k_immct_conformation(const float4 *__rest
                                                             const float4 *__restrict__ v,
                                                             const int32_t *__restrict__ mat,
                                                             const float *__restrict__ rs,
+                                                            const float4 *__restrict__ xmr,
                                                             const uint32_t *__restrict__ nbr,
                                                             const uint32_t *__restrict__ cnt, uint32_t n,
                                                             uint32_t stride, ImmK q, const float *__restrict__ fpoly,
@@ -465,14 +475,15 @@ __global__ __launch_bounds__(256) void k_immct_conformation(const float4 *__rest
     const float4 vi = v[i];
     const uint32_t nn = cnt[i] & ~kDeadBit;
     float L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    // a neighbour's signed rest density: > 0 exactly for the fluid ones
-    struct D { float4 x, v; float r; };
+    // a neighbour: (x_j, m_j / signed rest density), > 0 exactly for the
+    // fluid ones; and v_j
+    struct D { float4 x, v; };
     for_each_neighbor(
-        nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(x, j), ldg(v, j), ldg(rs, j)}; },
+        nbr, stride, i, nn, [&](uint32_t j) { return D{ldg(xmr, j), ldg(v, j)}; },
         [&](const D &d) {
-            if (!(d.r > 0.0f)) return;
+            if (!(d.x.w > 0.0f)) return;
             float dx = xi.x - d.x.x, dy = xi.y - d.x.y, dz = xi.z - d.x.z;
-            float f = cubic_grad_factor(dx * dx + dy * dy + dz * dz, q.k) * (d.x.w / d.r);
+            float f = cubic_grad_factor(dx * dx + dy * dy + dz * dz, q.k) * d.x.w;
             float gx = dx * f, gy = dy * f, gz = dz * f;
             float ux = d.v.x - vi.x, uy = d.v.y - vi.y, uz = d.v.z - vi.z;
             L[0] = fmaf(ux, gx, L[0]); L[1] = fmaf(ux, gy, L[1]); L[2] = fmaf(ux, gz, L[2]);
@@ -805,10 +816,14 @@ __global__ void k_imm_phase_update(float4 *__restrict__ x, float4 *__restrict__ 
 
 // The carried per-slot state into the new slot order, one pass and one read
 // of the permutation: phase fractions and phase velocities, the rest density
-// and its signed copy rs that the neighbour passes gather (+rest for a fluid
-// slot, -rest for a static one, rest >= 0, so one 4-byte gather gives a
-// neighbour's rest density and whether it is fluid), the colour and, for
-// IMM-CT (Q_src non-null), Q - I.  mat is already in the new order.
+// and its signed copy rs (+rest for a fluid slot, -rest for a static one,
+// rest >= 0), the colour and, for IMM-CT (Q_src non-null), Q - I.  x, v and
+// mat are already in the new order.  It also writes the records the
+// neighbour passes of the tick gather, constant until the positions move:
+//   xmr = (x, m / rs): position, and mass over the signed rest density,
+//         > 0 exactly for a fluid slot;
+//   aux = (V, polymer fraction, material bits, 0), and the polymer fraction
+//         alone in fpoly (IMM-CT: phase pp; 0 otherwise).
 template <int MP>
 __global__ void k_imm_gather_state(const uint32_t *__restrict__ perm, const int32_t *__restrict__ mat, uint32_t n,
                                    uint32_t stride, uint32_t P, const float *__restrict__ frac_src,
@@ -816,29 +831,35 @@ __global__ void k_imm_gather_state(const uint32_t *__restrict__ perm, const int3
                                    float4 *__restrict__ u_dst, const float *__restrict__ rest_src,
                                    float *__restrict__ rest_dst, float *__restrict__ rs,
                                    const float4 *__restrict__ col_src, float4 *__restrict__ col_dst,
-                                   const float *__restrict__ Q_src, float *__restrict__ Q_dst) {
+                                   const float *__restrict__ Q_src, float *__restrict__ Q_dst,
+                                   const float4 *__restrict__ x, const float4 *__restrict__ v, int pp,
+                                   float *__restrict__ fpoly, float4 *__restrict__ xmr, float4 *__restrict__ aux) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t o = perm[i];
+    float fp = 0.0f;
 #pragma unroll
     for (int k = 0; k < MP; ++k) {
         if (k >= (int)P) break;
-        frac_dst[(size_t)k * stride + i] = frac_src[(size_t)k * stride + o];
+        const float f = frac_src[(size_t)k * stride + o];
+        frac_dst[(size_t)k * stride + i] = f;
         u_dst[(size_t)k * stride + i] = u_src[(size_t)k * stride + o];
+        if (k == pp) fp = f;
     }
     const float r = rest_src[o];
+    const int32_t mi = mat[i];
+    const float rsi = mi == VTP_MAT_FLUID ? r : -r;
     rest_dst[i] = r;
-    rs[i] = mat[i] == VTP_MAT_FLUID ? r : -r;
+    rs[i] = rsi;
     col_dst[i] = col_src[o];
     if (Q_src) {
 #pragma unroll
         for (int a = 0; a < 9; ++a) Q_dst[(size_t)a * stride + i] = Q_src[(size_t)a * stride + o];
     }
-}
-__global__ void k_extract_phase(const float *__restrict__ frac, float *__restrict__ out, uint32_t n, uint32_t stride,
-                                int k) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = frac[(size_t)k * stride + i];
+    const float4 xi = x[i];
+    xmr[i] = make_float4(xi.x, xi.y, xi.z, xi.w / rsi);
+    aux[i] = make_float4(v[i].w, fp, __int_as_float(mi), 0.0f);
+    fpoly[i] = fp;
 }
 
 // ------------------------------------------------------------------ host side
@@ -891,6 +912,8 @@ void imm_initialize(Ctx &c) {
     c.flag.alloc(st);
     c.xveff.alloc(st);
     c.rs.alloc(st);
+    c.xmr.alloc(st);
+    c.aux.alloc(st);
     c.cd.alloc(st);
     c.fpoly.alloc(st);
     c.vis_ct.alloc(st);
@@ -932,7 +955,8 @@ static void imm_step_impl(Ctx &c) {
     common_reorder(c, c.X[0].p, c.X[1].p, nullptr, c.V[0].p, c.V[1].p, nullptr);
     hipLaunchKernelGGL(k_imm_gather_state<MP>, dim3(nb), dim3(256), 0, c.stream, u.perm.p, c.mat.p, n, st, P,
                        c.frac[0].p, c.frac[1].p, c.uphase[0].p, c.uphase[1].p, c.rest2[0].p, c.rest2[1].p, c.rs.p,
-                       c.color4[0].p, c.color4[1].p, q.ct ? c.Q[0].p : nullptr, c.Q[1].p);
+                       c.color4[0].p, c.color4[1].p, q.ct ? c.Q[0].p : nullptr, c.Q[1].p, c.X[1].p, c.V[1].p,
+                       q.ct ? q.pp : -1, c.fpoly.p, c.xmr.p, c.aux.p);
     c.timer.end(c.stream);
     std::swap(c.X[0].p, c.X[1].p);
     std::swap(c.V[0].p, c.V[1].p);
@@ -945,8 +969,6 @@ static void imm_step_impl(Ctx &c) {
     float *rest = c.rest2[0].p;
     Ph ph = phases(c);
     u.build_neighbors(x, c.stream, c.timer);
-    if (q.ct)
-        hipLaunchKernelGGL(k_extract_phase, dim3(nb), dim3(256), 0, c.stream, ph.frac, c.fpoly.p, n, st, q.pp);
     // workgroup size per neighbour pass (the tuner's choice, else "imm_pass")
     auto shape = [&](const char *pass) {
         auto it = c.block_of.find(pass);
@@ -957,7 +979,7 @@ static void imm_step_impl(Ctx &c) {
                s_ten = shape("imm_tension_phase_vel"), s_flux = shape("imm_fluxes");
     c.timer.begin(c.stream, "imm_precompute");
     hipLaunchKernelGGL(k_imm_precompute, s_pre.first, s_pre.second, 0, c.stream, x, v, c.vadv.p, c.mat.p, u.nbr.p, u.cnt.p, n,
-                       st, q, c.fpoly.p, c.cr.p, c.alpha.p, c.cd.p, c.g1.p, c.obj_of.p, c.sconst.p);
+                       st, q, c.aux.p, c.cr.p, c.alpha.p, c.cd.p, c.g1.p, c.obj_of.p, c.sconst.p);
     c.timer.end(c.stream);
     df_pressure_solve(c, x, 0);
     c.timer.begin(c.stream, "imm_pressure_to_phases");
@@ -965,18 +987,18 @@ static void imm_step_impl(Ctx &c) {
                        c.mat.p, rest, c.cd.p, n, q, ph, c.scal.p, 0, nullptr);
     c.timer.end(c.stream);
     c.timer.begin(c.stream, "imm_vis_normal");
-    hipLaunchKernelGGL(k_imm_vis_normal<MP>, s_vis.first, s_vis.second, 0, c.stream, x, v, c.mat.p, rest, c.rs.p,
+    hipLaunchKernelGGL(k_imm_vis_normal<MP>, s_vis.first, s_vis.second, 0, c.stream, x, v, c.mat.p, rest, c.xmr.p,
                        c.cd.p, u.nbr.p, u.cnt.p, n, st, q, ph, c.normal.p);
     c.timer.end(c.stream);
     c.timer.begin(c.stream, "imm_tension_phase_vel");
     hipLaunchKernelGGL(k_imm_tension_phase_vel<MP>, s_ten.first, s_ten.second, 0, c.stream, x, v, c.vadv.p, c.mat.p, rest,
-                       c.rs.p, c.normal.p, u.nbr.p, u.cnt.p, n, st, q, ph, c.scal.p);
+                       c.normal.p, u.nbr.p, u.cnt.p, n, st, q, ph, c.scal.p);
     c.timer.end(c.stream);
     if (q.ct) {
         c.timer.begin(c.stream, "immct_conformation");
         const auto s_conf = shape("immct_conformation"), s_str = shape("immct_stress_acc");
-        hipLaunchKernelGGL(k_immct_conformation, s_conf.first, s_conf.second, 0, c.stream, x, v, c.mat.p, c.rs.p, u.nbr.p,
-                           u.cnt.p, n, st, q, c.fpoly.p, c.Q[0].p, c.Lgrad.p, c.stress.p, c.vis_ct.p, c.srec.p, c.scal.p);
+        hipLaunchKernelGGL(k_immct_conformation, s_conf.first, s_conf.second, 0, c.stream, x, v, c.mat.p, c.rs.p, c.xmr.p,
+                           u.nbr.p, u.cnt.p, n, st, q, c.fpoly.p, c.Q[0].p, c.Lgrad.p, c.stress.p, c.vis_ct.p, c.srec.p, c.scal.p);
         c.timer.end(c.stream);
         c.timer.begin(c.stream, "immct_stress_acc");
         hipLaunchKernelGGL(k_immct_stress_acc<MP>, s_str.first, s_str.second, 0, c.stream, x, v, c.vadv.p, c.mat.p, rest,

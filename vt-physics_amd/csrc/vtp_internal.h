@@ -364,6 +364,8 @@ struct Ctx {
     DevBuf<float4> sconst;      // DFSPH/IMM solves: (cr, alpha, V, object id bits) per slot, alpha = 0 static
     DevBuf<float4> xveff;       // IMM: (x, V) of an unflagged fluid slot, (x, 0) otherwise (phase fluxes)
     DevBuf<float> rs;           // IMM: rest density of a fluid slot, its negation for a static one
+    DevBuf<float4> xmr;         // IMM: (x, m / rs) per slot, > 0 exactly for fluid (per tick)
+    DevBuf<float4> aux;         // IMM: (V, polymer fraction, material bits, 0) per slot (per tick)
     DevBuf<float> phase_rho, phase_vis;
     DevBuf<float4> phase_color;
     DevBuf<float4> color;
