@@ -636,13 +636,8 @@ __global__ __launch_bounds__(256) void k_imm_fluxes(const float4 *__restrict__ x
             if (fi0) return;   // flagged in an earlier round: its zero fluxes stand
             const uint32_t nn = cnt[i] & ~kDeadBit;
             bool dirty = false;
-            for (uint32_t t = 0; t < nn && !dirty; t += 8) {
-                uint32_t j[8];
-#pragma unroll
-                for (uint32_t b = 0; b < 8; ++b) j[b] = nbr_entry(nbr, stride, i, min(t + b, nn - 1));
-#pragma unroll
-                for (uint32_t b = 0; b < 8; ++b) dirty |= flag[j[b]] == tag;
-            }
+            for_each_neighbor(nbr, stride, i, nn, [&](uint32_t j) { return ldg(flag, j); },
+                              [&](uint32_t f) { dirty |= f == tag; });
             if (!dirty) return;
         }
     }
